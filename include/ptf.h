@@ -1,0 +1,158 @@
+/*
+ * libptf_hip -- C ABI of the MI355X TracerAdvectionDiffusion time step.
+ *
+ * Every entry point returns an int status (PTF_OK == 0); ptf_last_error()
+ * returns the message of the last failure on the calling thread.  No C++
+ * exception crosses this boundary.  Handles are not thread-safe; each
+ * problem, plan and ensemble owns one HIP stream (or uses the one given with
+ * *_set_stream) and every call is ordered on it.
+ *
+ * Layouts are the reference's (Julia column-major, x fastest):
+ *   physical  R-field : nx*ny*nz reals
+ *   spectral  S-field : (nx/2+1)*ny*nz complex, interleaved re/im
+ * Host-side arrays crossing the ABI are always FP64 (double / interleaved
+ * double pairs), whatever the precision the problem computes in.
+ *
+ * Reference interfaces replaced (PassiveTracerFlows.jl v0.9.1,
+ * src/traceradvectiondiffusion.jl abbreviated S; FourierFlows.jl 0.10.x [FF],
+ * not vendored in the reference, cited through its call sites):
+ */
+#ifndef PTF_H
+#define PTF_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  PTF_OK = 0,
+  PTF_ERR_INVALID = 1,     /* bad argument: null handle, unsupported size, ... */
+  PTF_ERR_DEVICE = 2,      /* HIP runtime failure                              */
+  PTF_ERR_NOMEM = 3,       /* device allocation failed                         */
+  PTF_ERR_UNSUPPORTED = 4  /* valid request outside what this build implements */
+};
+
+enum { PTF_FP64 = 0, PTF_MIXED = 1 };            /* arithmetic of the bulk fields   */
+enum { PTF_RK4 = 0, PTF_ETDRK4 = 1 };             /* stepper="RK4" / "ETDRK4"        */
+enum {
+  PTF_FLOW_STEADY = 0,    /* u,v,w sampled once on the grid (S:426-452)              */
+  PTF_FLOW_PER_STEP = 1,  /* u,v,w re-uploaded by the caller at every step boundary, */
+                          /* i.e. evaluated at clock.t (S:701, S:718, S:737)         */
+  PTF_FLOW_SPECTRAL = 2   /* built-in: psi_hat(t) = psi0_hat exp(-i omega t),        */
+                          /* u = -d_y psi, v = d_x psi, synthesised on the GPU at     */
+                          /* clock.t once per step (2D only)                         */
+};
+
+typedef struct ptf_desc {
+  int32_t ndim;          /* 1, 2 or 3                                                */
+  int32_t precision;     /* PTF_FP64 or PTF_MIXED                                    */
+  int64_t n[3];          /* nx, ny, nz (unused trailing entries ignored)             */
+  double L[3];           /* Lx, Ly, Lz                                               */
+  double kappa;          /* x-diffusivity kappa                                      */
+  double eta;            /* y-diffusivity eta (reference default eta = kappa)        */
+  double iota;           /* z-diffusivity iota (reference default iota = kappa)      */
+  double kappah;         /* hyperdiffusivity kappa_h                                 */
+  int32_t nkappah;       /* hyperdiffusivity order n_kappa_h (Krsq^n)                */
+  int32_t stepper;       /* PTF_RK4 or PTF_ETDRK4                                    */
+  double dt;             /* time step                                                */
+  int32_t flow_kind;     /* PTF_FLOW_*                                               */
+  int32_t reserved;
+} ptf_desc;
+
+typedef struct ptf_problem ptf_problem;
+typedef struct ptf_plan ptf_plan;
+typedef struct ptf_ensemble ptf_ensemble;
+
+/* ---- library / device ---------------------------------------------------- */
+const char* ptf_last_error(void);
+int ptf_version(int32_t* major, int32_t* minor);
+/* Select the device for the calling thread ([FF] Device / GPU() at S:505, S:646). */
+int ptf_device_init(int32_t ordinal);
+int ptf_device_synchronize(void);
+
+/* ---- device buffers (the arrays of the ROCm device; [FF] zeros(dev, T, dims),
+ *      @devzeros at S:505, S:646 and device_array(dev)(c) at S:847) ---------- */
+int ptf_buffer_alloc(uint64_t nbytes, void** dptr);
+int ptf_buffer_free(void* dptr);
+int ptf_buffer_upload(void* dptr, const void* host, uint64_t nbytes);
+int ptf_buffer_download(void* host, const void* dptr, uint64_t nbytes);
+
+/* ---- FFT plans: grid.rfftplan with mul!/ldiv! (S:20, S:698-783, S:818, S:847) */
+/* ndim/n as the grid; precision PTF_FP64 (double) or PTF_MIXED (float).        */
+int ptf_plan_create(int32_t ndim, const int64_t n[3], int32_t precision, ptf_plan** out);
+int ptf_plan_destroy(ptf_plan* plan);
+int ptf_plan_set_stream(ptf_plan* plan, void* hip_stream);
+/* mul!(out, rfftplan, in): unnormalised R2C; device pointers; input preserved. */
+int ptf_rfft(ptf_plan* plan, const void* in_real, void* out_complex);
+/* ldiv!(out, rfftplan, in): C2R scaled by 1/(nx*ny*nz); input preserved (a
+ * superset of FFTW's destroy-input contract); only the real parts of the
+ * kx = 0 and kx = nx/2 entries are used after the y/z inverse passes.        */
+int ptf_irfft(ptf_plan* plan, const void* in_complex, void* out_real);
+
+/* ---- problems: TracerAdvectionDiffusion.Problem (S:143-216) + the
+ *      [FF] FourierFlows.Problem/Clock/RK4TimeStepper/ETDRK4TimeStepper ---- */
+int ptf_problem_create(const ptf_desc* desc, ptf_problem** out);
+int ptf_problem_destroy(ptf_problem* prob);
+int ptf_problem_set_stream(ptf_problem* prob, void* hip_stream);
+int ptf_problem_get_stream(ptf_problem* prob, void** hip_stream);
+/* Steady fields (S:426-452) or the per-step refresh of a time-varying flow.
+ * Host FP64 R-fields; v, w may be NULL beyond ndim. */
+int ptf_set_flow(ptf_problem* prob, const double* u, const double* v, const double* w);
+/* PTF_FLOW_SPECTRAL: psi0 (interleaved complex S-field) and omega (real S-field). */
+int ptf_set_spectral_flow(ptf_problem* prob, const double* psih0, const double* omega);
+/* set_c!(prob, c) (S:844-852, S:872): sol = R2C(c); host FP64 R-field. */
+int ptf_set_c(ptf_problem* prob, const double* c);
+/* stepforward!(prob, nsteps) [FF]: clock.t += dt, clock.step += 1 after each step. */
+int ptf_step(ptf_problem* prob, int64_t nsteps);
+int ptf_get_clock(ptf_problem* prob, double* t, int64_t* step);
+int ptf_set_clock(ptf_problem* prob, double t, int64_t step);
+/* prob.sol read/write (host interleaved FP64 S-field). */
+int ptf_get_sol(ptf_problem* prob, double* sol);
+int ptf_set_sol(ptf_problem* prob, const double* sol);
+/* updatevars!(prob) (S:815-821, S:837): c = C2R(sol) into a host FP64 R-field. */
+int ptf_updatevars(ptf_problem* prob, double* c);
+/* calcN!(N, sol, t, clock, vars, params, grid) (S:695-786) on host arrays: the
+ * operator plug-in evaluated once with the current flow (FP64 host S-fields). */
+int ptf_calcN(ptf_problem* prob, const double* sol, double* N);
+/* The diagonal linear operator L of Equation(params, grid) (S:502-554), host
+ * real S-shaped array. */
+int ptf_get_L(ptf_problem* prob, double* L);
+/* FP64 diagnostics: mass = sum(c) = Re sol[0]; variance = mean((c - mean c)^2)
+ * by a compensated Parseval sum over the half spectrum. */
+int ptf_diagnostics(ptf_problem* prob, double* mass, double* variance);
+/* Device pointer of sol (S-field of the problem's precision). */
+int ptf_problem_sol_ptr(ptf_problem* prob, void** dptr);
+/* Pick the fastest compiled kernel variant for every launch slot. */
+int ptf_autotune(ptf_problem* prob);
+/* Kernel-slot introspection for measurement: number of launches per step,
+ * name of slot i, and the average duration of slot i over reps launches
+ * (HIP events on the problem's stream).  Timing launches write only scratch
+ * fields: the state is unchanged. */
+int ptf_problem_num_slots(ptf_problem* prob, int32_t* nslots);
+int ptf_problem_slot_name(ptf_problem* prob, int32_t slot, char* buf, int32_t buflen);
+int ptf_problem_time_slot(ptf_problem* prob, int32_t slot, int32_t reps, double* avg_ms);
+
+/* ---- ensembles: many independent 2D problems, one graph-captured launch
+ *      sequence per step (no reference counterpart: a batch of
+ *      TracerAdvectionDiffusion.Problem(dev, TwoDAdvectingFlow(...)) with
+ *      steady flows and RK4) ------------------------------------------------ */
+int ptf_ensemble_create(const ptf_desc* descs, int32_t count, ptf_ensemble** out);
+int ptf_ensemble_destroy(ptf_ensemble* ens);
+int ptf_ensemble_set_stream(ptf_ensemble* ens, void* hip_stream);
+int ptf_ensemble_set_flow(ptf_ensemble* ens, int32_t member, const double* u, const double* v);
+int ptf_ensemble_set_c(ptf_ensemble* ens, int32_t member, const double* c);
+int ptf_ensemble_get_c(ptf_ensemble* ens, int32_t member, double* c);
+int ptf_ensemble_get_sol(ptf_ensemble* ens, int32_t member, double* sol);
+/* Advance every member nsteps steps; use_graph != 0 replays a captured hipGraph. */
+int ptf_ensemble_step(ptf_ensemble* ens, int64_t nsteps, int32_t use_graph);
+int ptf_ensemble_autotune(ptf_ensemble* ens);
+int ptf_ensemble_num_slots(ptf_ensemble* ens, int32_t* nslots);
+int ptf_ensemble_slot_name(ptf_ensemble* ens, int32_t slot, char* buf, int32_t buflen);
+int ptf_ensemble_time_slot(ptf_ensemble* ens, int32_t slot, int32_t reps, double* avg_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PTF_H */

@@ -1,0 +1,105 @@
+"""Build libptf_hip.so for gfx950 in-tree (passivetracerflows.jl_amd/lib/).
+
+    python passivetracerflows.jl_amd/build.py [--jobs N] [--incremental]
+
+The kernel registry (csrc/ptf_registry.hip) is compiled once per
+(precision, kernel kind) so the units build in parallel; csrc/ptf_api.hip is
+the host runtime.  All HIP code is built with packed-FP32 instructions off.
+"""
+from __future__ import annotations
+
+import argparse
+import concurrent.futures as cf
+import os
+import subprocess
+import sys
+import time
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+LIBDIR = os.path.join(HERE, "lib")
+OBJDIR = os.path.join(HERE, "_build")
+INCLUDE = os.path.join(os.path.dirname(HERE), "include")
+LIBNAME = "libptf_hip.so"
+
+ARCH = os.environ.get("PTF_OFFLOAD_ARCH", "gfx950")
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+COMMON = [
+    f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-I", CSRC, "-I", INCLUDE, "-Wno-unused-command-line-argument",
+]
+KINDS = [0, 1, 2, 3, 4, 5, 10, 11, 12, 13, 14]
+PRECS = [("double", "f64"), ("float", "f32")]
+HEADERS = ["ptf_device.hpp", "ptf_kernels.hpp"]
+API_HEADERS = HEADERS + ["ptf_setup.hpp"]
+
+
+def _units():
+    units = []
+    for t, tn in PRECS:
+        for k in KINDS:
+            obj = os.path.join(OBJDIR, f"reg_{tn}_{k}.o")
+            cmd = [HIPCC, *COMMON, f"-DPTF_T={t}", f"-DPTF_TNAME={tn}", f"-DPTF_KIND={k}",
+                   "-c", os.path.join(CSRC, "ptf_registry.hip"), "-o", obj]
+            units.append((obj, cmd, ["ptf_registry.hip", *HEADERS]))
+    obj = os.path.join(OBJDIR, "ptf_api.o")
+    units.append((obj, [HIPCC, *COMMON, "-c", os.path.join(CSRC, "ptf_api.hip"), "-o", obj],
+                  ["ptf_api.hip", *API_HEADERS]))
+    return units
+
+
+def _stale(obj, deps):
+    if not os.path.exists(obj):
+        return True
+    t = os.path.getmtime(obj)
+    return any(os.path.getmtime(os.path.join(CSRC, d)) > t for d in deps) or \
+        os.path.getmtime(os.path.join(INCLUDE, "ptf.h")) > t
+
+
+def _run(cmd):
+    t0 = time.time()
+    p = subprocess.run(cmd, capture_output=True, text=True)
+    err = "\n".join(l for l in (p.stderr or "").splitlines()
+                    if "packed-fp32-ops" not in l and l.strip())
+    return p.returncode, err, time.time() - t0
+
+
+def build(jobs: int | None = None, incremental: bool = False, verbose: bool = True) -> str:
+    os.makedirs(OBJDIR, exist_ok=True)
+    os.makedirs(LIBDIR, exist_ok=True)
+    jobs = jobs or min(8, os.cpu_count() or 4)
+    units = _units()
+    todo = [(o, c) for o, c, d in units if not incremental or _stale(o, d)]
+    t0 = time.time()
+    with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
+        futs = {ex.submit(_run, c): o for o, c in todo}
+        for f in cf.as_completed(futs):
+            rc, err, dt = f.result()
+            if rc != 0:
+                raise RuntimeError(f"hipcc failed for {futs[f]}:\n{err}")
+            if verbose:
+                print(f"  built {os.path.basename(futs[f])} in {dt:.1f}s", flush=True)
+    lib = os.path.join(LIBDIR, LIBNAME)
+    tmp = lib + ".tmp"
+    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp,
+           *[o for o, _, _ in units]]
+    rc, err, _ = _run(cmd)
+    if rc != 0:
+        raise RuntimeError(f"link failed:\n{err}")
+    os.replace(tmp, lib)
+    if verbose:
+        print(f"libptf_hip: {lib} ({time.time() - t0:.1f}s)", flush=True)
+    return lib
+
+
+if __name__ == "__main__":
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--jobs", type=int, default=None)
+    ap.add_argument("--incremental", action="store_true")
+    a = ap.parse_args()
+    try:
+        build(a.jobs, a.incremental)
+    except RuntimeError as e:
+        print(e, file=sys.stderr)
+        sys.exit(1)

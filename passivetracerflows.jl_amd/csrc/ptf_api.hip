@@ -1,0 +1,1461 @@
+// Host runtime of libptf_hip: the C ABI declared in include/ptf.h.
+//
+// A "group" is B independent problems that share one grid (a single problem
+// is a group with B = 1; an ensemble is one group per distinct grid size).
+// Every group owns its state fields, scratch fields, flows, FFT twiddle tables
+// and a per-stage list of kernel launches ("slots") whose arguments are fixed
+// at creation, so one step is a plain walk over the slot lists.
+#include "ptf.h"
+#include "ptf_setup.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+using namespace ptf;
+
+// ---------------------------------------------------------------------------
+// errors
+// ---------------------------------------------------------------------------
+static thread_local std::string g_err;
+
+static int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIPCHK(x)                                                                          \
+  do {                                                                                     \
+    hipError_t e_ = (x);                                                                   \
+    if (e_ != hipSuccess) return fail(PTF_ERR_DEVICE, std::string(#x) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+#define CHK(x)                  \
+  do {                          \
+    int r_ = (x);               \
+    if (r_ != PTF_OK) return r_; \
+  } while (0)
+
+// ---------------------------------------------------------------------------
+// kernel registry (ptf_registry.hip, one object per precision and kind)
+// ---------------------------------------------------------------------------
+enum {
+  K_COL_FWD = 0, K_COL_INV = 1, K_COL_D1 = 2, K_COL_D2 = 3, K_COL_STAGE = 4, K_COL_FLOWSYN = 5,
+  K_ROW_C2R = 10, K_ROW_R2C = 11, K_ROW_ADV1 = 12, K_ROW_ADV2 = 13, K_ROW_ADV3 = 14
+};
+
+#define PTF_DECL(t, k) extern "C" const void* ptf_reg_##t##_##k(int, int, int);
+#define PTF_DECL_ALL(t) \
+  PTF_DECL(t, 0) PTF_DECL(t, 1) PTF_DECL(t, 2) PTF_DECL(t, 3) PTF_DECL(t, 4) PTF_DECL(t, 5) \
+  PTF_DECL(t, 10) PTF_DECL(t, 11) PTF_DECL(t, 12) PTF_DECL(t, 13) PTF_DECL(t, 14)
+PTF_DECL_ALL(f64)
+PTF_DECL_ALL(f32)
+
+typedef const void* (*RegFn)(int, int, int);
+
+static RegFn reg_fn(bool dbl, int kind) {
+#define PTF_CASE(k) case k: return dbl ? ptf_reg_f64_##k : ptf_reg_f32_##k;
+  switch (kind) {
+    PTF_CASE(0) PTF_CASE(1) PTF_CASE(2) PTF_CASE(3) PTF_CASE(4) PTF_CASE(5)
+    PTF_CASE(10) PTF_CASE(11) PTF_CASE(12) PTF_CASE(13) PTF_CASE(14)
+    default: return nullptr;
+  }
+#undef PTF_CASE
+}
+
+static const char* kind_name(int kind) {
+  switch (kind) {
+    case K_COL_FWD: return "col_fft_fwd";
+    case K_COL_INV: return "col_fft_inv";
+    case K_COL_D1: return "col_inv_deriv1";
+    case K_COL_D2: return "col_inv_deriv2";
+    case K_COL_STAGE: return "col_fwd_stage";
+    case K_COL_FLOWSYN: return "col_flow_synth";
+    case K_ROW_C2R: return "row_c2r";
+    case K_ROW_R2C: return "row_r2c";
+    case K_ROW_ADV1: return "row_advect1";
+    case K_ROW_ADV2: return "row_advect2";
+    case K_ROW_ADV3: return "row_advect3";
+  }
+  return "?";
+}
+
+static const int kRadices[] = {16, 8, 4};
+static const int kColWidths[] = {16, 8, 32, 4, 2};
+static const int kRowWidths[] = {8, 16, 4, 2, 1};
+
+// ---------------------------------------------------------------------------
+// launch slots
+// ---------------------------------------------------------------------------
+struct Slot {
+  std::string name;
+  bool dbl = true;
+  int kind = 0;
+  int N = 0;            // axis length (columns) or nx (rows)
+  int R = 0, W = 0;     // radix, columns/rows per block
+  const void* fn = nullptr;
+  long long lines = 0;  // ncols (columns) or nrows (rows)
+  int other = 1;        // grid.y for columns
+  int members = 1;
+  std::vector<unsigned char> args;  // ColArgs<T> or RowArgs<T>
+  bool writes_state = false;        // launch modifies sol (timing must restore it)
+
+  bool is_col() const { return kind < 10; }
+  int threads() const { return is_col() ? W * (N / R) : W * ((N / 2) / R); }
+  dim3 grid() const {
+    if (is_col()) return dim3((unsigned)((lines + W - 1) / W), (unsigned)other, (unsigned)members);
+    return dim3((unsigned)((lines + W - 1) / W), (unsigned)members, 1);
+  }
+};
+
+static bool choose_variant(Slot& s, int R, int W) {
+  RegFn f = reg_fn(s.dbl, s.kind);
+  if (!f) return false;
+  const void* p = f(s.N, R, W);
+  if (!p) return false;
+  s.R = R;
+  s.W = W;
+  s.fn = p;
+  return true;
+}
+
+// default variant: first valid (R, W) in preference order
+static int pick_default(Slot& s) {
+  const int* widths = s.is_col() ? kColWidths : kRowWidths;
+  for (int R : kRadices) {
+    for (int wi = 0; wi < 5; ++wi) {
+      int W = widths[wi];
+      if (!s.is_col() && s.lines < W && W > 1) continue;  // no idle rows
+      if (choose_variant(s, R, W)) return PTF_OK;
+    }
+  }
+  // fall back to any width
+  for (int R : kRadices)
+    for (int wi = 0; wi < 5; ++wi)
+      if (choose_variant(s, R, widths[wi])) return PTF_OK;
+  return fail(PTF_ERR_UNSUPPORTED, std::string("no kernel variant for ") + kind_name(s.kind) +
+                                        " with line length " + std::to_string(s.N));
+}
+
+static int launch(const Slot& s, hipStream_t stream) {
+  void* argv[1] = {(void*)s.args.data()};
+  HIPCHK(hipLaunchKernel(s.fn, s.grid(), dim3(s.threads()), argv, 0, stream));
+  return PTF_OK;
+}
+
+template <typename A>
+static void set_args(Slot& s, const A& a) {
+  s.args.resize(sizeof(A));
+  std::memcpy(s.args.data(), &a, sizeof(A));
+}
+template <typename A>
+static A& args_of(Slot& s) {
+  return *reinterpret_cast<A*>(s.args.data());
+}
+
+// ---------------------------------------------------------------------------
+// twiddles: tw[m] = exp(-2 pi i m / L), m < L, from an octant-reduced long
+// double sin/cos rounded once to the working precision
+// ---------------------------------------------------------------------------
+static void cs_exact(long n, long L, long double* c, long double* s) {
+  n %= L;
+  if (n < 0) n += L;
+  if (2 * n > L) {  // (pi, 2 pi): conjugate
+    cs_exact(L - n, L, c, s);
+    *s = -*s;
+    return;
+  }
+  if (4 * n > L) {  // (pi/2, pi]
+    cs_exact(L / 2 - n, L, c, s);
+    *c = -*c;
+    return;
+  }
+  if (8 * n > L) {  // (pi/4, pi/2]
+    long double cc, ss;
+    cs_exact(L / 4 - n, L, &cc, &ss);
+    *c = ss;
+    *s = cc;
+    return;
+  }
+  const long double pi = 3.141592653589793238462643383279502884L;
+  const long double th = 2.0L * pi * (long double)n / (long double)L;
+  *c = cosl(th);
+  *s = sinl(th);
+}
+
+template <typename T>
+static std::vector<cpx<T>> twiddle_table(int L) {
+  std::vector<cpx<T>> tw(L);
+  for (int m = 0; m < L; ++m) {
+    long double c, s;
+    if (L % 8 == 0) {
+      cs_exact(m, L, &c, &s);
+    } else {
+      const long double pi = 3.141592653589793238462643383279502884L;
+      c = cosl(2.0L * pi * m / L);
+      s = sinl(2.0L * pi * m / L);
+    }
+    tw[m].re = (T)(double)c;
+    tw[m].im = (T)(double)(-s);
+  }
+  return tw;
+}
+
+// ---------------------------------------------------------------------------
+// device memory helpers
+// ---------------------------------------------------------------------------
+struct DevBuf {
+  void* p = nullptr;
+  size_t n = 0;
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+};
+
+static int dalloc(DevBuf& b, size_t nbytes) {
+  if (b.p) {
+    (void)hipFree(b.p);
+    b.p = nullptr;
+  }
+  if (nbytes == 0) nbytes = 16;
+  hipError_t e = hipMalloc(&b.p, nbytes);
+  if (e != hipSuccess) {
+    b.p = nullptr;
+    (void)hipGetLastError();
+    return fail(PTF_ERR_NOMEM, "hipMalloc of " + std::to_string(nbytes) + " bytes failed: " +
+                                   hipGetErrorString(e));
+  }
+  b.n = nbytes;
+  return PTF_OK;
+}
+
+static bool is_pow2(long long n) { return n > 0 && (n & (n - 1)) == 0; }
+
+// ---------------------------------------------------------------------------
+// group of B problems on one grid
+// ---------------------------------------------------------------------------
+struct Group {
+  int ndim = 1, nx = 1, ny = 1, nz = 1, nxh = 1;
+  int B = 1;
+  bool dbl = true;
+  size_t esz = 8;
+  int stepper = PTF_RK4;
+  int flow_kind = PTF_FLOW_STEADY;
+  long long nS = 0, nR = 0;  // elements per member
+  double dt = 0.01;
+  int nkappah = 0;
+  double Ls[3] = {0, 0, 0}, ks[3] = {0, 0, 0};
+  std::vector<OpParams> ops;
+
+  DevBuf d_ops, sol, stg, stg2, acc, n1, n2, w[3], flow[3], d_L, etd[6], tw[3], psih0, omega;
+  DevBuf phys, spec, red, backup;  // set_c/updatevars/calcN staging, reductions, timing backup
+  std::vector<Slot> stages[4];
+  std::vector<Slot> flowsyn;
+  // plan slots (B = 1 view, pointers set per call)
+  Slot p_r2c, p_c2r, p_yf, p_yi, p_zf, p_zi;
+  hipStream_t stream = nullptr;
+};
+
+template <typename T>
+static GridInfo grid_info(const Group& g) {
+  GridInfo gi;
+  gi.ndim = g.ndim;
+  gi.nx = g.nx;
+  gi.ny = g.ny;
+  gi.nz = g.nz;
+  gi.ksx = g.ks[0];
+  gi.ksy = g.ks[1];
+  gi.ksz = g.ks[2];
+  return gi;
+}
+
+// wavenumber scale as the grid's rfftfreq(n, 2 pi/dx) computes it: (2 pi/dx)/n
+static double kscale(int n, double L) {
+  const double dx = L / n;
+  const double fs = 2.0 * M_PI / dx;
+  return fs / n;
+}
+
+static int validate_desc(const ptf_desc* d) {
+  if (!d) return fail(PTF_ERR_INVALID, "desc is NULL");
+  if (d->ndim < 1 || d->ndim > 3) return fail(PTF_ERR_INVALID, "ndim must be 1, 2 or 3");
+  if (d->precision != PTF_FP64 && d->precision != PTF_MIXED)
+    return fail(PTF_ERR_INVALID, "precision must be PTF_FP64 or PTF_MIXED");
+  if (d->stepper != PTF_RK4 && d->stepper != PTF_ETDRK4)
+    return fail(PTF_ERR_INVALID, "stepper must be PTF_RK4 or PTF_ETDRK4");
+  if (d->flow_kind < PTF_FLOW_STEADY || d->flow_kind > PTF_FLOW_SPECTRAL)
+    return fail(PTF_ERR_INVALID, "unknown flow_kind");
+  if (d->flow_kind == PTF_FLOW_SPECTRAL && d->ndim != 2)
+    return fail(PTF_ERR_UNSUPPORTED, "the built-in spectral flow is 2D only");
+  if (d->nkappah < 0) return fail(PTF_ERR_INVALID, "nkappah must be >= 0");
+  if (!(d->dt > 0) || !std::isfinite(d->dt)) return fail(PTF_ERR_INVALID, "dt must be positive");
+  for (int a = 0; a < d->ndim; ++a) {
+    const long long n = d->n[a];
+    if (n <= 0 || (n % 2)) return fail(PTF_ERR_INVALID, "grid sizes must be positive and even");
+    if (!(d->L[a] > 0)) return fail(PTF_ERR_INVALID, "domain lengths must be positive");
+    if (!is_pow2(n))
+      return fail(PTF_ERR_UNSUPPORTED, "grid size " + std::to_string(n) +
+                                           " is not a power of two (supported: powers of two)");
+    if (a == 0 && (n < 16 || n > 4096))
+      return fail(PTF_ERR_UNSUPPORTED, "nx must be in [16, 4096], got " + std::to_string(n));
+    if (a > 0 && (n < 8 || n > 2048))
+      return fail(PTF_ERR_UNSUPPORTED, "ny and nz must be in [8, 2048], got " + std::to_string(n));
+  }
+  return PTF_OK;
+}
+
+// ----- slot builders ---------------------------------------------------------
+template <typename T>
+static ColArgs<T> col_base(const Group& g, int axis) {
+  ColArgs<T> a;
+  std::memset(&a, 0, sizeof(a));
+  a.tw = (const cpx<T>*)g.tw[axis].p;
+  a.twstride = 1;
+  a.ncols = g.nxh;
+  a.mstride = g.nS;
+  a.kscale = g.ks[axis];
+  a.axis = axis;
+  if (axis == 1) {
+    a.stride = g.nxh;
+    a.ostride = (long long)g.nxh * g.ny;
+  } else {
+    a.stride = (long long)g.nxh * g.ny;
+    a.ostride = g.nxh;
+  }
+  a.kscale_x = g.ks[0];
+  return a;
+}
+
+template <typename T>
+static RowArgs<T> row_base(const Group& g) {
+  RowArgs<T> a;
+  std::memset(&a, 0, sizeof(a));
+  a.tw = (const cpx<T>*)g.tw[0].p;
+  a.twstride = 1;
+  a.nrows = (long long)g.ny * g.nz;
+  a.smstride = g.nS;
+  a.pmstride = g.nR;
+  a.scale = 1.0 / ((double)g.nx * g.ny * g.nz);
+  a.kscale_x = g.ks[0];
+  a.ny = g.ny;
+  return a;
+}
+
+static Slot make_slot(const Group& g, int kind, int axis, int members) {
+  Slot s;
+  s.dbl = g.dbl;
+  s.kind = kind;
+  s.members = members;
+  if (kind < 10) {
+    s.N = axis == 1 ? g.ny : g.nz;
+    s.lines = g.nxh;
+    s.other = axis == 1 ? g.nz : g.ny;
+  } else {
+    s.N = g.nx;
+    s.lines = (long long)g.ny * g.nz;
+  }
+  return s;
+}
+
+template <typename T>
+static StageArgs<T> stage_args(const Group& g, int stage) {
+  StageArgs<T> st;
+  std::memset(&st, 0, sizeof(st));
+  st.kind = g.stepper == PTF_RK4 ? EPI_RK4 : EPI_ETDRK4;
+  st.stage = stage;
+  st.sol = (cpx<T>*)g.sol.p;
+  st.stg = (cpx<T>*)g.stg.p;
+  st.stg2 = (cpx<T>*)g.stg2.p;
+  st.acc = (cpx<T>*)g.acc.p;
+  st.n1 = (cpx<T>*)g.n1.p;
+  st.n2 = (cpx<T>*)g.n2.p;
+  st.E = (const double*)g.etd[0].p;
+  st.E2 = (const double*)g.etd[1].p;
+  st.zeta = (const double*)g.etd[2].p;
+  st.alpha = (const double*)g.etd[3].p;
+  st.beta = (const double*)g.etd[4].p;
+  st.gam = (const double*)g.etd[5].p;
+  st.dt = g.dt;
+  st.op = (const OpParams*)g.d_ops.p;
+  st.nkappah = g.nkappah;
+  st.g = grid_info<T>(g);
+  st.mstride = g.nS;
+  return st;
+}
+
+// Launch list of one calcN! evaluation of `in`, ending in `st` (a stage
+// epilogue or a plain store to `out`).
+template <typename T>
+static int build_calcN(Group& g, const cpx<T>* in, const StageArgs<T>& st, cpx<T>* out,
+                       std::vector<Slot>& list, const std::string& prefix) {
+  cpx<T>* w0 = (cpx<T>*)g.w[0].p;
+  cpx<T>* w1 = (cpx<T>*)g.w[1].p;
+  cpx<T>* w2 = (cpx<T>*)g.w[2].p;
+  const T* u = (const T*)g.flow[0].p;
+  const T* v = (const T*)g.flow[1].p;
+  const T* w = (const T*)g.flow[2].p;
+  if (g.ndim == 1) {
+    Slot s = make_slot(g, K_ROW_ADV1, 0, g.B);
+    RowArgs<T> a = row_base<T>(g);
+    a.in[0] = in;
+    a.flow[0] = u;
+    a.out = out;
+    a.st = st;
+    set_args(s, a);
+    s.name = prefix + "x_advect_stage";
+    s.writes_state = st.kind != EPI_STORE && st.stage == 3;
+    CHK(pick_default(s));
+    list.push_back(s);
+    return PTF_OK;
+  }
+  if (g.ndim == 2) {
+    Slot s1 = make_slot(g, K_COL_D1, 1, g.B);
+    ColArgs<T> a1 = col_base<T>(g, 1);
+    a1.in0 = in;
+    a1.out0 = w0;
+    a1.out1 = w1;
+    set_args(s1, a1);
+    s1.name = prefix + "y_inverse_deriv";
+    CHK(pick_default(s1));
+    list.push_back(s1);
+
+    Slot s2 = make_slot(g, K_ROW_ADV2, 0, g.B);
+    RowArgs<T> a2 = row_base<T>(g);
+    a2.in[0] = w0;
+    a2.in[1] = w1;
+    a2.flow[0] = u;
+    a2.flow[1] = v;
+    a2.out = w0;
+    a2.st.kind = EPI_STORE;
+    set_args(s2, a2);
+    s2.name = prefix + "x_advect";
+    CHK(pick_default(s2));
+    list.push_back(s2);
+
+    Slot s3 = make_slot(g, K_COL_STAGE, 1, g.B);
+    ColArgs<T> a3 = col_base<T>(g, 1);
+    a3.in0 = w0;
+    a3.out0 = out;
+    a3.st = st;
+    set_args(s3, a3);
+    s3.name = prefix + "y_forward_stage";
+    s3.writes_state = st.kind != EPI_STORE && st.stage == 3;
+    CHK(pick_default(s3));
+    list.push_back(s3);
+    return PTF_OK;
+  }
+  // 3D
+  Slot s1 = make_slot(g, K_COL_D1, 2, g.B);
+  ColArgs<T> a1 = col_base<T>(g, 2);
+  a1.in0 = in;
+  a1.out0 = w0;
+  a1.out1 = w1;
+  set_args(s1, a1);
+  s1.name = prefix + "z_inverse_deriv";
+  CHK(pick_default(s1));
+  list.push_back(s1);
+
+  Slot s2 = make_slot(g, K_COL_D2, 1, g.B);
+  ColArgs<T> a2 = col_base<T>(g, 1);
+  a2.in0 = w0;
+  a2.in1 = w1;
+  a2.out0 = w0;
+  a2.out1 = w2;
+  a2.out2 = w1;
+  set_args(s2, a2);
+  s2.name = prefix + "y_inverse_deriv2";
+  CHK(pick_default(s2));
+  list.push_back(s2);
+
+  Slot s3 = make_slot(g, K_ROW_ADV3, 0, g.B);
+  RowArgs<T> a3 = row_base<T>(g);
+  a3.in[0] = w0;   // IFFT_yz(c)        -> multiplied by i kr in the kernel
+  a3.in[1] = w2;   // IFFT_y(i l IFFT_z(c))
+  a3.in[2] = w1;   // IFFT_y(IFFT_z(i m c))
+  a3.flow[0] = u;
+  a3.flow[1] = v;
+  a3.flow[2] = w;
+  a3.out = w0;
+  a3.st.kind = EPI_STORE;
+  set_args(s3, a3);
+  s3.name = prefix + "x_advect";
+  CHK(pick_default(s3));
+  list.push_back(s3);
+
+  Slot s4 = make_slot(g, K_COL_FWD, 1, g.B);
+  ColArgs<T> a4 = col_base<T>(g, 1);
+  a4.in0 = w0;
+  a4.out0 = w0;
+  set_args(s4, a4);
+  s4.name = prefix + "y_forward";
+  CHK(pick_default(s4));
+  list.push_back(s4);
+
+  Slot s5 = make_slot(g, K_COL_STAGE, 2, g.B);
+  ColArgs<T> a5 = col_base<T>(g, 2);
+  a5.in0 = w0;
+  a5.out0 = out;
+  a5.st = st;
+  set_args(s5, a5);
+  s5.name = prefix + "z_forward_stage";
+  s5.writes_state = st.kind != EPI_STORE && st.stage == 3;
+  CHK(pick_default(s5));
+  list.push_back(s5);
+  return PTF_OK;
+}
+
+template <typename T>
+static int build_group_slots(Group& g) {
+  for (int s = 0; s < 4; ++s) {
+    g.stages[s].clear();
+    const cpx<T>* in;
+    if (g.stepper == PTF_RK4) {
+      in = (const cpx<T>*)(s == 0 ? g.sol.p : g.stg.p);
+    } else {
+      in = (const cpx<T>*)(s == 0 ? g.sol.p : s == 1 ? g.stg.p : g.stg2.p);
+    }
+    StageArgs<T> st = stage_args<T>(g, s);
+    CHK(build_calcN<T>(g, in, st, nullptr, g.stages[s], "stage" + std::to_string(s + 1) + "_"));
+  }
+  g.flowsyn.clear();
+  if (g.flow_kind == PTF_FLOW_SPECTRAL) {
+    Slot s1 = make_slot(g, K_COL_FLOWSYN, 1, 1);
+    ColArgs<T> a1 = col_base<T>(g, 1);
+    a1.psih0 = (const cpx<double>*)g.psih0.p;
+    a1.omega = (const double*)g.omega.p;
+    a1.out0 = (cpx<T>*)g.w[0].p;
+    a1.out1 = (cpx<T>*)g.w[1].p;
+    set_args(s1, a1);
+    s1.name = "flow_y_synthesis";
+    CHK(pick_default(s1));
+    g.flowsyn.push_back(s1);
+    for (int c = 0; c < 2; ++c) {
+      Slot s2 = make_slot(g, K_ROW_C2R, 0, 1);
+      RowArgs<T> a2 = row_base<T>(g);
+      a2.in[0] = (const cpx<T>*)g.w[c].p;
+      a2.rout = (T*)g.flow[c].p;
+      set_args(s2, a2);
+      s2.name = c == 0 ? "flow_x_c2r_u" : "flow_x_c2r_v";
+      CHK(pick_default(s2));
+      g.flowsyn.push_back(s2);
+    }
+  }
+  // plan slots (single member, pointers patched per call)
+  g.p_r2c = make_slot(g, K_ROW_R2C, 0, 1);
+  set_args(g.p_r2c, row_base<T>(g));
+  g.p_r2c.name = "plan_x_r2c";
+  CHK(pick_default(g.p_r2c));
+  g.p_c2r = make_slot(g, K_ROW_C2R, 0, 1);
+  set_args(g.p_c2r, row_base<T>(g));
+  g.p_c2r.name = "plan_x_c2r";
+  CHK(pick_default(g.p_c2r));
+  if (g.ndim >= 2) {
+    g.p_yf = make_slot(g, K_COL_FWD, 1, 1);
+    set_args(g.p_yf, col_base<T>(g, 1));
+    g.p_yf.name = "plan_y_fwd";
+    CHK(pick_default(g.p_yf));
+    g.p_yi = make_slot(g, K_COL_INV, 1, 1);
+    set_args(g.p_yi, col_base<T>(g, 1));
+    g.p_yi.name = "plan_y_inv";
+    CHK(pick_default(g.p_yi));
+  }
+  if (g.ndim == 3) {
+    g.p_zf = make_slot(g, K_COL_FWD, 2, 1);
+    set_args(g.p_zf, col_base<T>(g, 2));
+    g.p_zf.name = "plan_z_fwd";
+    CHK(pick_default(g.p_zf));
+    g.p_zi = make_slot(g, K_COL_INV, 2, 1);
+    set_args(g.p_zi, col_base<T>(g, 2));
+    g.p_zi.name = "plan_z_inv";
+    CHK(pick_default(g.p_zi));
+  }
+  return PTF_OK;
+}
+
+template <typename T>
+static int upload_twiddles(Group& g) {
+  const int lens[3] = {g.nx, g.ny, g.nz};
+  for (int a = 0; a < g.ndim; ++a) {
+    std::vector<cpx<T>> tw = twiddle_table<T>(lens[a]);
+    CHK(dalloc(g.tw[a], tw.size() * sizeof(cpx<T>)));
+    HIPCHK(hipMemcpy(g.tw[a].p, tw.data(), tw.size() * sizeof(cpx<T>), hipMemcpyHostToDevice));
+  }
+  return PTF_OK;
+}
+
+static int group_init(Group& g, const ptf_desc* descs, int count) {
+  const ptf_desc& d = descs[0];
+  g.ndim = d.ndim;
+  g.nx = (int)d.n[0];
+  g.ny = d.ndim >= 2 ? (int)d.n[1] : 1;
+  g.nz = d.ndim >= 3 ? (int)d.n[2] : 1;
+  g.nxh = g.nx / 2 + 1;
+  g.B = count;
+  g.dbl = d.precision == PTF_FP64;
+  g.esz = g.dbl ? 8 : 4;
+  g.stepper = d.stepper;
+  g.flow_kind = d.flow_kind;
+  g.nS = (long long)g.nxh * g.ny * g.nz;
+  g.nR = (long long)g.nx * g.ny * g.nz;
+  g.dt = d.dt;
+  g.nkappah = d.nkappah;
+  for (int a = 0; a < 3; ++a) {
+    g.Ls[a] = a < d.ndim ? d.L[a] : 1.0;
+    g.ks[a] = a < d.ndim ? kscale((int)d.n[a], d.L[a]) : 0.0;
+  }
+  g.ops.resize(count);
+  for (int i = 0; i < count; ++i) {
+    g.ops[i].kappa = descs[i].kappa;
+    g.ops[i].eta = descs[i].eta;
+    g.ops[i].iota = descs[i].iota;
+    g.ops[i].kappah = descs[i].kappah;
+  }
+  const size_t cs = 2 * g.esz;  // complex element bytes
+  const size_t S = (size_t)g.nS * g.B * cs;
+  const size_t Rb = (size_t)g.nR * g.B * g.esz;
+  CHK(dalloc(g.d_ops, sizeof(OpParams) * count));
+  HIPCHK(hipMemcpy(g.d_ops.p, g.ops.data(), sizeof(OpParams) * count, hipMemcpyHostToDevice));
+  CHK(dalloc(g.sol, S));
+  CHK(dalloc(g.stg, S));
+  CHK(dalloc(g.acc, S));
+  if (g.stepper == PTF_ETDRK4) {
+    CHK(dalloc(g.stg2, S));
+    CHK(dalloc(g.n1, S));
+    CHK(dalloc(g.n2, S));
+  }
+  for (int i = 0; i < (g.ndim == 3 ? 3 : 2); ++i) CHK(dalloc(g.w[i], S));
+  for (int i = 0; i < g.ndim; ++i) CHK(dalloc(g.flow[i], Rb));
+  HIPCHK(hipMemset(g.sol.p, 0, S));
+  for (int i = 0; i < g.ndim; ++i) HIPCHK(hipMemset(g.flow[i].p, 0, Rb));
+  CHK(dalloc(g.phys, (size_t)g.nR * g.esz));
+  CHK(dalloc(g.spec, (size_t)g.nS * cs));
+  CHK(dalloc(g.red, 2 * sizeof(double) * 1024));
+  if (g.flow_kind == PTF_FLOW_SPECTRAL) {
+    CHK(dalloc(g.psih0, (size_t)g.nS * 2 * sizeof(double)));
+    CHK(dalloc(g.omega, (size_t)g.nS * sizeof(double)));
+    HIPCHK(hipMemset(g.psih0.p, 0, (size_t)g.nS * 2 * sizeof(double)));
+    HIPCHK(hipMemset(g.omega.p, 0, (size_t)g.nS * sizeof(double)));
+  }
+  // linear operator (FP64) for get_L and the ETDRK4 coefficients
+  {
+    std::vector<double> L((size_t)g.nS * g.B);
+    for (int b = 0; b < g.B; ++b) {
+      const OpParams& p = g.ops[b];
+      for (int kz = 0; kz < g.nz; ++kz)
+        for (int ky = 0; ky < g.ny; ++ky)
+          for (int kx = 0; kx < g.nxh; ++kx) {
+            const double kr = kx * g.ks[0];
+            const double kr2 = kr * kr;
+            double v;
+            if (g.ndim == 1) {
+              v = -p.kappa * kr2 - p.kappah * std::pow(kr2, g.nkappah);
+            } else {
+              const int iy = ky < g.ny / 2 ? ky : ky - g.ny;
+              const double l = iy * g.ks[1];
+              const double l2 = l * l;
+              if (g.ndim == 2) {
+                v = -p.kappa * kr2 - p.eta * l2 - p.kappah * std::pow(kr2 + l2, g.nkappah);
+              } else {
+                const int iz = kz < g.nz / 2 ? kz : kz - g.nz;
+                const double m = iz * g.ks[2];
+                const double m2 = m * m;
+                v = -p.kappa * kr2 - p.eta * l2 - p.iota * m2 -
+                    p.kappah * std::pow(kr2 + l2 + m2, g.nkappah);
+              }
+            }
+            L[(size_t)b * g.nS + kx + (size_t)g.nxh * (ky + (size_t)g.ny * kz)] = v;
+          }
+    }
+    CHK(dalloc(g.d_L, L.size() * sizeof(double)));
+    HIPCHK(hipMemcpy(g.d_L.p, L.data(), L.size() * sizeof(double), hipMemcpyHostToDevice));
+  }
+  if (g.stepper == PTF_ETDRK4) {
+    const long long n = g.nS * g.B;
+    for (int i = 0; i < 6; ++i) CHK(dalloc(g.etd[i], (size_t)n * sizeof(double)));
+    const int th = 256;
+    hipLaunchKernelGGL(etd_coeff_kernel, dim3((unsigned)((n + th - 1) / th)), dim3(th), 0, 0,
+                       (const double*)g.d_L.p, n, g.dt, 32, 1.0, (double*)g.etd[0].p,
+                       (double*)g.etd[1].p, (double*)g.etd[2].p, (double*)g.etd[3].p,
+                       (double*)g.etd[4].p, (double*)g.etd[5].p);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipDeviceSynchronize());
+  }
+  if (g.dbl) {
+    CHK(upload_twiddles<double>(g));
+    CHK(build_group_slots<double>(g));
+  } else {
+    CHK(upload_twiddles<float>(g));
+    CHK(build_group_slots<float>(g));
+  }
+  return PTF_OK;
+}
+
+// ----- plan-style transforms on one member -------------------------------------
+template <typename T>
+static int group_rfft(Group& g, const void* in_real, void* out_complex, hipStream_t st) {
+  {
+    RowArgs<T>& a = args_of<RowArgs<T>>(g.p_r2c);
+    a.rin = (const T*)in_real;
+    a.out = (cpx<T>*)out_complex;
+    CHK(launch(g.p_r2c, st));
+  }
+  if (g.ndim >= 2) {
+    ColArgs<T>& a = args_of<ColArgs<T>>(g.p_yf);
+    a.in0 = (const cpx<T>*)out_complex;
+    a.out0 = (cpx<T>*)out_complex;
+    CHK(launch(g.p_yf, st));
+  }
+  if (g.ndim == 3) {
+    ColArgs<T>& a = args_of<ColArgs<T>>(g.p_zf);
+    a.in0 = (const cpx<T>*)out_complex;
+    a.out0 = (cpx<T>*)out_complex;
+    CHK(launch(g.p_zf, st));
+  }
+  return PTF_OK;
+}
+
+template <typename T>
+static int group_irfft(Group& g, const void* in_complex, void* out_real, hipStream_t st) {
+  const cpx<T>* src = (const cpx<T>*)in_complex;
+  if (g.ndim == 3) {
+    ColArgs<T>& a = args_of<ColArgs<T>>(g.p_zi);
+    a.in0 = src;
+    a.out0 = (cpx<T>*)g.spec.p;
+    CHK(launch(g.p_zi, st));
+    src = (const cpx<T>*)g.spec.p;
+  }
+  if (g.ndim >= 2) {
+    ColArgs<T>& a = args_of<ColArgs<T>>(g.p_yi);
+    a.in0 = src;
+    a.out0 = (cpx<T>*)g.spec.p;
+    CHK(launch(g.p_yi, st));
+    src = (const cpx<T>*)g.spec.p;
+  }
+  RowArgs<T>& a = args_of<RowArgs<T>>(g.p_c2r);
+  a.in[0] = src;
+  a.rout = (T*)out_real;
+  CHK(launch(g.p_c2r, st));
+  return PTF_OK;
+}
+
+static int g_rfft(Group& g, const void* in, void* out, hipStream_t st) {
+  return g.dbl ? group_rfft<double>(g, in, out, st) : group_rfft<float>(g, in, out, st);
+}
+static int g_irfft(Group& g, const void* in, void* out, hipStream_t st) {
+  return g.dbl ? group_irfft<double>(g, in, out, st) : group_irfft<float>(g, in, out, st);
+}
+
+// host FP64 <-> device working precision
+static int upload_real(Group& g, void* dst, const double* src, size_t n, hipStream_t st) {
+  if (g.dbl) {
+    HIPCHK(hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyHostToDevice, st));
+  } else {
+    std::vector<float> tmp(n);
+    for (size_t i = 0; i < n; ++i) tmp[i] = (float)src[i];
+    HIPCHK(hipMemcpyAsync(dst, tmp.data(), n * sizeof(float), hipMemcpyHostToDevice, st));
+    HIPCHK(hipStreamSynchronize(st));
+  }
+  return PTF_OK;
+}
+
+static int download_real(Group& g, double* dst, const void* src, size_t n, hipStream_t st) {
+  if (g.dbl) {
+    HIPCHK(hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+  } else {
+    std::vector<float> tmp(n);
+    HIPCHK(hipMemcpyAsync(tmp.data(), src, n * sizeof(float), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    for (size_t i = 0; i < n; ++i) dst[i] = (double)tmp[i];
+  }
+  return PTF_OK;
+}
+
+static int group_step_launches(Group& g, hipStream_t st, double t) {
+  if (!g.flowsyn.empty()) {
+    Slot& s = g.flowsyn[0];
+    if (g.dbl) args_of<ColArgs<double>>(s).t = t;
+    else args_of<ColArgs<float>>(s).t = t;
+    for (auto& fs : g.flowsyn) CHK(launch(fs, st));
+  }
+  for (int s = 0; s < 4; ++s)
+    for (auto& sl : g.stages[s]) CHK(launch(sl, st));
+  return PTF_OK;
+}
+
+// all slots of one step, in order (flow synthesis first)
+static std::vector<Slot*> step_slots(Group& g) {
+  std::vector<Slot*> v;
+  for (auto& s : g.flowsyn) v.push_back(&s);
+  for (int s = 0; s < 4; ++s)
+    for (auto& sl : g.stages[s]) v.push_back(&sl);
+  return v;
+}
+
+static int time_slot(Group& g, Slot& s, int reps, hipStream_t st, double* avg_ms) {
+  if (reps < 1) reps = 1;
+  const size_t S = (size_t)g.nS * g.B * 2 * g.esz;
+  if (s.writes_state) {
+    if (!g.backup.p) CHK(dalloc(g.backup, S));
+    HIPCHK(hipMemcpyAsync(g.backup.p, g.sol.p, S, hipMemcpyDeviceToDevice, st));
+  }
+  CHK(launch(s, st));  // warm-up
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  HIPCHK(hipEventRecord(e0, st));
+  for (int r = 0; r < reps; ++r) CHK(launch(s, st));
+  HIPCHK(hipEventRecord(e1, st));
+  HIPCHK(hipEventSynchronize(e1));
+  float ms = 0;
+  HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (s.writes_state) {
+    HIPCHK(hipMemcpyAsync(g.sol.p, g.backup.p, S, hipMemcpyDeviceToDevice, st));
+    HIPCHK(hipStreamSynchronize(st));
+  }
+  *avg_ms = (double)ms / reps;
+  return PTF_OK;
+}
+
+// autotune: for every distinct (kind, N, lines) pick the fastest valid (R, W)
+static int autotune_group(Group& g, hipStream_t st) {
+  std::vector<Slot*> all = step_slots(g);
+  std::vector<std::string> done;
+  for (Slot* s : all) {
+    const std::string key = std::to_string(s->kind) + ":" + std::to_string(s->N) + ":" +
+                            std::to_string(s->lines) + ":" + std::to_string(s->members);
+    if (std::find(done.begin(), done.end(), key) != done.end()) continue;
+    done.push_back(key);
+    const int* widths = s->is_col() ? kColWidths : kRowWidths;
+    int bestR = s->R, bestW = s->W;
+    double best = 1e30;
+    for (int R : kRadices) {
+      for (int wi = 0; wi < 5; ++wi) {
+        Slot trial = *s;
+        if (!choose_variant(trial, R, widths[wi])) continue;
+        double ms = 0;
+        CHK(time_slot(g, trial, 5, st, &ms));
+        if (ms < best) {
+          best = ms;
+          bestR = R;
+          bestW = widths[wi];
+        }
+      }
+    }
+    for (Slot* o : all) {
+      const std::string k2 = std::to_string(o->kind) + ":" + std::to_string(o->N) + ":" +
+                             std::to_string(o->lines) + ":" + std::to_string(o->members);
+      if (k2 == key) choose_variant(*o, bestR, bestW);
+    }
+  }
+  return PTF_OK;
+}
+
+// ---------------------------------------------------------------------------
+// public handles
+// ---------------------------------------------------------------------------
+struct ptf_problem {
+  ptf_desc d;
+  Group g;
+  double t = 0.0;
+  long long step = 0;
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  bool flow_set = false;
+  ~ptf_problem() {
+    if (own_stream && stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+struct ptf_plan {
+  Group g;
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  ~ptf_plan() {
+    if (own_stream && stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+struct ptf_ensemble {
+  std::vector<std::unique_ptr<Group>> groups;
+  std::vector<std::pair<int, int>> where;  // member -> (group, index in group)
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  std::vector<hipStream_t> gstreams;
+  std::vector<hipEvent_t> events;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  ~ptf_ensemble() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    for (auto e : events) (void)hipEventDestroy(e);
+    for (auto s : gstreams) (void)hipStreamDestroy(s);
+    if (own_stream && stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+extern "C" {
+
+const char* ptf_last_error(void) { return g_err.c_str(); }
+
+int ptf_version(int32_t* major, int32_t* minor) {
+  if (major) *major = 0;
+  if (minor) *minor = 1;
+  return PTF_OK;
+}
+
+int ptf_device_init(int32_t ordinal) {
+  int n = 0;
+  HIPCHK(hipGetDeviceCount(&n));
+  if (ordinal < 0 || ordinal >= n)
+    return fail(PTF_ERR_INVALID, "device ordinal " + std::to_string(ordinal) + " out of range (" +
+                                     std::to_string(n) + " devices)");
+  HIPCHK(hipSetDevice(ordinal));
+  return PTF_OK;
+}
+
+int ptf_device_synchronize(void) {
+  HIPCHK(hipDeviceSynchronize());
+  return PTF_OK;
+}
+
+int ptf_buffer_alloc(uint64_t nbytes, void** dptr) {
+  if (!dptr) return fail(PTF_ERR_INVALID, "dptr is NULL");
+  hipError_t e = hipMalloc(dptr, nbytes ? nbytes : 16);
+  if (e != hipSuccess) {
+    *dptr = nullptr;
+    (void)hipGetLastError();
+    return fail(PTF_ERR_NOMEM, std::string("hipMalloc failed: ") + hipGetErrorString(e));
+  }
+  return PTF_OK;
+}
+
+int ptf_buffer_free(void* dptr) {
+  if (dptr) HIPCHK(hipFree(dptr));
+  return PTF_OK;
+}
+
+int ptf_buffer_upload(void* dptr, const void* host, uint64_t nbytes) {
+  if (!dptr || !host) return fail(PTF_ERR_INVALID, "NULL pointer");
+  HIPCHK(hipMemcpy(dptr, host, nbytes, hipMemcpyHostToDevice));
+  return PTF_OK;
+}
+
+int ptf_buffer_download(void* host, const void* dptr, uint64_t nbytes) {
+  if (!dptr || !host) return fail(PTF_ERR_INVALID, "NULL pointer");
+  HIPCHK(hipMemcpy(host, dptr, nbytes, hipMemcpyDeviceToHost));
+  return PTF_OK;
+}
+
+// ----- plans -----------------------------------------------------------------
+int ptf_plan_create(int32_t ndim, const int64_t n[3], int32_t precision, ptf_plan** out) {
+  if (!out || !n) return fail(PTF_ERR_INVALID, "NULL argument");
+  *out = nullptr;
+  ptf_desc d;
+  std::memset(&d, 0, sizeof(d));
+  d.ndim = ndim;
+  d.precision = precision;
+  for (int a = 0; a < 3; ++a) {
+    d.n[a] = a < ndim ? n[a] : 1;
+    d.L[a] = 2 * M_PI;
+  }
+  d.dt = 1.0;
+  CHK(validate_desc(&d));
+  std::unique_ptr<ptf_plan> p(new ptf_plan());
+  CHK(group_init(p->g, &d, 1));
+  HIPCHK(hipStreamCreate(&p->stream));
+  p->own_stream = true;
+  *out = p.release();
+  return PTF_OK;
+}
+
+int ptf_plan_destroy(ptf_plan* plan) {
+  delete plan;
+  return PTF_OK;
+}
+
+int ptf_plan_set_stream(ptf_plan* plan, void* s) {
+  if (!plan) return fail(PTF_ERR_INVALID, "plan is NULL");
+  if (plan->own_stream && plan->stream) (void)hipStreamDestroy(plan->stream);
+  plan->own_stream = false;
+  plan->stream = (hipStream_t)s;
+  return PTF_OK;
+}
+
+int ptf_rfft(ptf_plan* plan, const void* in_real, void* out_complex) {
+  if (!plan || !in_real || !out_complex) return fail(PTF_ERR_INVALID, "NULL argument");
+  CHK(g_rfft(plan->g, in_real, out_complex, plan->stream));
+  return PTF_OK;
+}
+
+int ptf_irfft(ptf_plan* plan, const void* in_complex, void* out_real) {
+  if (!plan || !in_complex || !out_real) return fail(PTF_ERR_INVALID, "NULL argument");
+  CHK(g_irfft(plan->g, in_complex, out_real, plan->stream));
+  return PTF_OK;
+}
+
+// ----- problems ----------------------------------------------------------------
+int ptf_problem_create(const ptf_desc* desc, ptf_problem** out) {
+  if (!out) return fail(PTF_ERR_INVALID, "out is NULL");
+  *out = nullptr;
+  CHK(validate_desc(desc));
+  std::unique_ptr<ptf_problem> p(new ptf_problem());
+  p->d = *desc;
+  CHK(group_init(p->g, desc, 1));
+  HIPCHK(hipStreamCreate(&p->stream));
+  p->own_stream = true;
+  *out = p.release();
+  return PTF_OK;
+}
+
+int ptf_problem_destroy(ptf_problem* prob) {
+  if (prob && prob->stream) (void)hipStreamSynchronize(prob->stream);
+  delete prob;
+  return PTF_OK;
+}
+
+int ptf_problem_set_stream(ptf_problem* prob, void* s) {
+  if (!prob) return fail(PTF_ERR_INVALID, "problem is NULL");
+  if (prob->own_stream && prob->stream) {
+    (void)hipStreamSynchronize(prob->stream);
+    (void)hipStreamDestroy(prob->stream);
+  }
+  prob->own_stream = false;
+  prob->stream = (hipStream_t)s;
+  return PTF_OK;
+}
+
+int ptf_problem_get_stream(ptf_problem* prob, void** s) {
+  if (!prob || !s) return fail(PTF_ERR_INVALID, "NULL argument");
+  *s = (void*)prob->stream;
+  return PTF_OK;
+}
+
+int ptf_set_flow(ptf_problem* prob, const double* u, const double* v, const double* w) {
+  if (!prob) return fail(PTF_ERR_INVALID, "problem is NULL");
+  if (prob->d.flow_kind == PTF_FLOW_SPECTRAL)
+    return fail(PTF_ERR_INVALID, "problem uses the built-in spectral flow; use ptf_set_spectral_flow");
+  const double* f[3] = {u, v, w};
+  Group& g = prob->g;
+  for (int i = 0; i < g.ndim; ++i) {
+    if (!f[i]) return fail(PTF_ERR_INVALID, "flow component " + std::to_string(i) + " is NULL");
+    CHK(upload_real(g, g.flow[i].p, f[i], (size_t)g.nR, prob->stream));
+  }
+  HIPCHK(hipStreamSynchronize(prob->stream));
+  prob->flow_set = true;
+  return PTF_OK;
+}
+
+int ptf_set_spectral_flow(ptf_problem* prob, const double* psih0, const double* omega) {
+  if (!prob || !psih0 || !omega) return fail(PTF_ERR_INVALID, "NULL argument");
+  if (prob->d.flow_kind != PTF_FLOW_SPECTRAL)
+    return fail(PTF_ERR_INVALID, "problem was not created with PTF_FLOW_SPECTRAL");
+  Group& g = prob->g;
+  HIPCHK(hipMemcpy(g.psih0.p, psih0, (size_t)g.nS * 2 * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(g.omega.p, omega, (size_t)g.nS * sizeof(double), hipMemcpyHostToDevice));
+  prob->flow_set = true;
+  return PTF_OK;
+}
+
+int ptf_set_c(ptf_problem* prob, const double* c) {
+  if (!prob || !c) return fail(PTF_ERR_INVALID, "NULL argument");
+  Group& g = prob->g;
+  CHK(upload_real(g, g.phys.p, c, (size_t)g.nR, prob->stream));
+  CHK(g_rfft(g, g.phys.p, g.sol.p, prob->stream));
+  HIPCHK(hipStreamSynchronize(prob->stream));
+  return PTF_OK;
+}
+
+int ptf_step(ptf_problem* prob, int64_t nsteps) {
+  if (!prob) return fail(PTF_ERR_INVALID, "problem is NULL");
+  if (nsteps < 0) return fail(PTF_ERR_INVALID, "nsteps must be >= 0");
+  for (int64_t i = 0; i < nsteps; ++i) {
+    CHK(group_step_launches(prob->g, prob->stream, prob->t));
+    prob->t += prob->g.dt;
+    prob->step += 1;
+  }
+  HIPCHK(hipGetLastError());
+  return PTF_OK;
+}
+
+int ptf_get_clock(ptf_problem* prob, double* t, int64_t* step) {
+  if (!prob) return fail(PTF_ERR_INVALID, "problem is NULL");
+  if (t) *t = prob->t;
+  if (step) *step = prob->step;
+  return PTF_OK;
+}
+
+int ptf_set_clock(ptf_problem* prob, double t, int64_t step) {
+  if (!prob) return fail(PTF_ERR_INVALID, "problem is NULL");
+  prob->t = t;
+  prob->step = step;
+  return PTF_OK;
+}
+
+static int download_spec(Group& g, double* dst, const void* src, hipStream_t st) {
+  return download_real(g, dst, src, (size_t)g.nS * 2, st);
+}
+static int upload_spec(Group& g, void* dst, const double* src, hipStream_t st) {
+  return upload_real(g, dst, src, (size_t)g.nS * 2, st);
+}
+
+int ptf_get_sol(ptf_problem* prob, double* sol) {
+  if (!prob || !sol) return fail(PTF_ERR_INVALID, "NULL argument");
+  return download_spec(prob->g, sol, prob->g.sol.p, prob->stream);
+}
+
+int ptf_set_sol(ptf_problem* prob, const double* sol) {
+  if (!prob || !sol) return fail(PTF_ERR_INVALID, "NULL argument");
+  CHK(upload_spec(prob->g, prob->g.sol.p, sol, prob->stream));
+  HIPCHK(hipStreamSynchronize(prob->stream));
+  return PTF_OK;
+}
+
+int ptf_updatevars(ptf_problem* prob, double* c) {
+  if (!prob || !c) return fail(PTF_ERR_INVALID, "NULL argument");
+  Group& g = prob->g;
+  CHK(g_irfft(g, g.sol.p, g.phys.p, prob->stream));
+  return download_real(g, c, g.phys.p, (size_t)g.nR, prob->stream);
+}
+
+}  // extern "C"
+
+template <typename T>
+static int calcN_impl(ptf_problem* prob, const double* sol, double* N) {
+  Group& g = prob->g;
+  hipStream_t st = prob->stream;
+  if (g.flow_kind == PTF_FLOW_SPECTRAL) {
+    args_of<ColArgs<T>>(g.flowsyn[0]).t = prob->t;
+    for (auto& fs : g.flowsyn) CHK(launch(fs, st));
+  }
+  CHK(upload_spec(g, g.stg.p, sol, st));
+  StageArgs<T> none;
+  std::memset(&none, 0, sizeof(none));
+  none.kind = EPI_STORE;
+  std::vector<Slot> list;
+  CHK(build_calcN<T>(g, (const cpx<T>*)g.stg.p, none, (cpx<T>*)g.acc.p, list, "calcN_"));
+  for (auto& s : list) CHK(launch(s, st));
+  return download_spec(g, N, g.acc.p, st);
+}
+
+extern "C" {
+
+int ptf_calcN(ptf_problem* prob, const double* sol, double* N) {
+  if (!prob || !sol || !N) return fail(PTF_ERR_INVALID, "NULL argument");
+  return prob->g.dbl ? calcN_impl<double>(prob, sol, N) : calcN_impl<float>(prob, sol, N);
+}
+
+int ptf_get_L(ptf_problem* prob, double* L) {
+  if (!prob || !L) return fail(PTF_ERR_INVALID, "NULL argument");
+  HIPCHK(hipMemcpy(L, prob->g.d_L.p, (size_t)prob->g.nS * sizeof(double), hipMemcpyDeviceToHost));
+  return PTF_OK;
+}
+
+static int diagnostics_group(Group& g, int member, hipStream_t st, double* mass, double* var) {
+  const size_t cs = 2 * g.esz;
+  const char* base = (const char*)g.sol.p + (size_t)member * g.nS * cs;
+  double s0re = 0, s0im = 0;
+  if (g.dbl) {
+    double v[2];
+    HIPCHK(hipMemcpyAsync(v, base, 16, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    s0re = v[0];
+    s0im = v[1];
+  } else {
+    float v[2];
+    HIPCHK(hipMemcpyAsync(v, base, 8, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    s0re = v[0];
+    s0im = v[1];
+  }
+  if (mass) *mass = s0re;
+  if (var) {
+    const int blocks = 1024;
+    if (g.dbl)
+      hipLaunchKernelGGL(variance_partial_kernel<double>, dim3(blocks), dim3(256), 0, st,
+                         (const cpx<double>*)base, g.nS, g.nxh, (double*)g.red.p);
+    else
+      hipLaunchKernelGGL(variance_partial_kernel<float>, dim3(blocks), dim3(256), 0, st,
+                         (const cpx<float>*)base, g.nS, g.nxh, (double*)g.red.p);
+    HIPCHK(hipGetLastError());
+    std::vector<double> part(2 * blocks);
+    HIPCHK(hipMemcpyAsync(part.data(), g.red.p, part.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    double sum = 0, comp = 0;
+    for (int i = 0; i < 2 * blocks; ++i) {
+      const double x = part[i];
+      const double t = sum + x;
+      comp += (std::fabs(sum) >= std::fabs(x)) ? (sum - t) + x : (x - t) + sum;
+      sum = t;
+    }
+    const double dc = s0re * s0re + s0im * s0im;
+    const double tot = (sum - dc) + comp;
+    const double N = (double)g.nR;
+    *var = tot / (N * N);
+  }
+  return PTF_OK;
+}
+
+int ptf_diagnostics(ptf_problem* prob, double* mass, double* variance) {
+  if (!prob) return fail(PTF_ERR_INVALID, "problem is NULL");
+  return diagnostics_group(prob->g, 0, prob->stream, mass, variance);
+}
+
+int ptf_problem_sol_ptr(ptf_problem* prob, void** dptr) {
+  if (!prob || !dptr) return fail(PTF_ERR_INVALID, "NULL argument");
+  *dptr = prob->g.sol.p;
+  return PTF_OK;
+}
+
+int ptf_autotune(ptf_problem* prob) {
+  if (!prob) return fail(PTF_ERR_INVALID, "problem is NULL");
+  return autotune_group(prob->g, prob->stream);
+}
+
+int ptf_problem_num_slots(ptf_problem* prob, int32_t* n) {
+  if (!prob || !n) return fail(PTF_ERR_INVALID, "NULL argument");
+  *n = (int32_t)step_slots(prob->g).size();
+  return PTF_OK;
+}
+
+static int slot_name_into(Slot& s, char* buf, int32_t buflen) {
+  if (!buf || buflen <= 0) return fail(PTF_ERR_INVALID, "bad buffer");
+  char tmp[256];
+  std::snprintf(tmp, sizeof(tmp), "%s [%s N=%d R=%d W=%d %s]", s.name.c_str(), kind_name(s.kind),
+                s.N, s.R, s.W, s.dbl ? "f64" : "f32");
+  std::strncpy(buf, tmp, (size_t)buflen - 1);
+  buf[buflen - 1] = 0;
+  return PTF_OK;
+}
+
+int ptf_problem_slot_name(ptf_problem* prob, int32_t slot, char* buf, int32_t buflen) {
+  if (!prob) return fail(PTF_ERR_INVALID, "problem is NULL");
+  auto v = step_slots(prob->g);
+  if (slot < 0 || slot >= (int)v.size()) return fail(PTF_ERR_INVALID, "slot out of range");
+  return slot_name_into(*v[slot], buf, buflen);
+}
+
+int ptf_problem_time_slot(ptf_problem* prob, int32_t slot, int32_t reps, double* avg_ms) {
+  if (!prob || !avg_ms) return fail(PTF_ERR_INVALID, "NULL argument");
+  auto v = step_slots(prob->g);
+  if (slot < 0 || slot >= (int)v.size()) return fail(PTF_ERR_INVALID, "slot out of range");
+  return time_slot(prob->g, *v[slot], reps, prob->stream, avg_ms);
+}
+
+// ----- ensembles ---------------------------------------------------------------
+int ptf_ensemble_create(const ptf_desc* descs, int32_t count, ptf_ensemble** out) {
+  if (!out || !descs || count <= 0) return fail(PTF_ERR_INVALID, "bad arguments");
+  *out = nullptr;
+  for (int i = 0; i < count; ++i) {
+    CHK(validate_desc(&descs[i]));
+    if (descs[i].flow_kind == PTF_FLOW_SPECTRAL)
+      return fail(PTF_ERR_UNSUPPORTED, "ensemble members take steady or per-step flows");
+  }
+  std::unique_ptr<ptf_ensemble> e(new ptf_ensemble());
+  e->where.resize(count);
+  // group members with identical grid, precision, stepper, dt and nkappah
+  std::vector<std::vector<int>> groups;
+  for (int i = 0; i < count; ++i) {
+    const ptf_desc& a = descs[i];
+    int gi = -1;
+    for (size_t k = 0; k < groups.size(); ++k) {
+      const ptf_desc& b = descs[groups[k][0]];
+      bool same = a.ndim == b.ndim && a.precision == b.precision && a.stepper == b.stepper &&
+                  a.dt == b.dt && a.nkappah == b.nkappah;
+      for (int x = 0; x < a.ndim && same; ++x) same = a.n[x] == b.n[x] && a.L[x] == b.L[x];
+      if (same) {
+        gi = (int)k;
+        break;
+      }
+    }
+    if (gi < 0) {
+      groups.push_back({});
+      gi = (int)groups.size() - 1;
+    }
+    e->where[i] = {gi, (int)groups[gi].size()};
+    groups[gi].push_back(i);
+  }
+  for (auto& members : groups) {
+    std::vector<ptf_desc> ds;
+    for (int i : members) ds.push_back(descs[i]);
+    std::unique_ptr<Group> g(new Group());
+    CHK(group_init(*g, ds.data(), (int)ds.size()));
+    e->groups.push_back(std::move(g));
+  }
+  HIPCHK(hipStreamCreate(&e->stream));
+  e->own_stream = true;
+  for (size_t k = 0; k < e->groups.size(); ++k) {
+    hipStream_t s;
+    HIPCHK(hipStreamCreate(&s));
+    e->gstreams.push_back(s);
+    hipEvent_t ev;
+    HIPCHK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    e->events.push_back(ev);
+  }
+  hipEvent_t ev;
+  HIPCHK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+  e->events.push_back(ev);
+  *out = e.release();
+  return PTF_OK;
+}
+
+int ptf_ensemble_destroy(ptf_ensemble* ens) {
+  if (ens && ens->stream) (void)hipStreamSynchronize(ens->stream);
+  delete ens;
+  return PTF_OK;
+}
+
+static void drop_graph(ptf_ensemble* e) {
+  if (e->exec) (void)hipGraphExecDestroy(e->exec);
+  if (e->graph) (void)hipGraphDestroy(e->graph);
+  e->exec = nullptr;
+  e->graph = nullptr;
+}
+
+int ptf_ensemble_set_stream(ptf_ensemble* ens, void* s) {
+  if (!ens) return fail(PTF_ERR_INVALID, "ensemble is NULL");
+  if (ens->own_stream && ens->stream) {
+    (void)hipStreamSynchronize(ens->stream);
+    (void)hipStreamDestroy(ens->stream);
+  }
+  ens->own_stream = false;
+  ens->stream = (hipStream_t)s;
+  drop_graph(ens);
+  return PTF_OK;
+}
+
+static int member_of(ptf_ensemble* e, int member, Group** g, int* idx) {
+  if (!e) return fail(PTF_ERR_INVALID, "ensemble is NULL");
+  if (member < 0 || member >= (int)e->where.size()) return fail(PTF_ERR_INVALID, "member out of range");
+  *g = e->groups[e->where[member].first].get();
+  *idx = e->where[member].second;
+  return PTF_OK;
+}
+
+int ptf_ensemble_set_flow(ptf_ensemble* ens, int32_t member, const double* u, const double* v) {
+  Group* g;
+  int i;
+  CHK(member_of(ens, member, &g, &i));
+  const double* f[3] = {u, v, nullptr};
+  for (int c = 0; c < g->ndim && c < 2; ++c) {
+    if (!f[c]) return fail(PTF_ERR_INVALID, "NULL flow component");
+    char* dst = (char*)g->flow[c].p + (size_t)i * g->nR * g->esz;
+    CHK(upload_real(*g, dst, f[c], (size_t)g->nR, ens->stream));
+  }
+  HIPCHK(hipStreamSynchronize(ens->stream));
+  return PTF_OK;
+}
+
+int ptf_ensemble_set_c(ptf_ensemble* ens, int32_t member, const double* c) {
+  Group* g;
+  int i;
+  CHK(member_of(ens, member, &g, &i));
+  if (!c) return fail(PTF_ERR_INVALID, "c is NULL");
+  CHK(upload_real(*g, g->phys.p, c, (size_t)g->nR, ens->stream));
+  char* dst = (char*)g->sol.p + (size_t)i * g->nS * 2 * g->esz;
+  CHK(g_rfft(*g, g->phys.p, dst, ens->stream));
+  HIPCHK(hipStreamSynchronize(ens->stream));
+  return PTF_OK;
+}
+
+int ptf_ensemble_get_c(ptf_ensemble* ens, int32_t member, double* c) {
+  Group* g;
+  int i;
+  CHK(member_of(ens, member, &g, &i));
+  if (!c) return fail(PTF_ERR_INVALID, "c is NULL");
+  char* src = (char*)g->sol.p + (size_t)i * g->nS * 2 * g->esz;
+  CHK(g_irfft(*g, src, g->phys.p, ens->stream));
+  return download_real(*g, c, g->phys.p, (size_t)g->nR, ens->stream);
+}
+
+int ptf_ensemble_get_sol(ptf_ensemble* ens, int32_t member, double* sol) {
+  Group* g;
+  int i;
+  CHK(member_of(ens, member, &g, &i));
+  if (!sol) return fail(PTF_ERR_INVALID, "sol is NULL");
+  char* src = (char*)g->sol.p + (size_t)i * g->nS * 2 * g->esz;
+  return download_spec(*g, sol, src, ens->stream);
+}
+
+static int ensemble_record_step(ptf_ensemble* e) {
+  hipStream_t s0 = e->stream;
+  const int ng = (int)e->groups.size();
+  hipEvent_t fork = e->events[ng];
+  HIPCHK(hipEventRecord(fork, s0));
+  for (int k = 0; k < ng; ++k) {
+    HIPCHK(hipStreamWaitEvent(e->gstreams[k], fork, 0));
+    CHK(group_step_launches(*e->groups[k], e->gstreams[k], 0.0));
+    HIPCHK(hipEventRecord(e->events[k], e->gstreams[k]));
+  }
+  for (int k = 0; k < ng; ++k) HIPCHK(hipStreamWaitEvent(s0, e->events[k], 0));
+  return PTF_OK;
+}
+
+int ptf_ensemble_step(ptf_ensemble* ens, int64_t nsteps, int32_t use_graph) {
+  if (!ens) return fail(PTF_ERR_INVALID, "ensemble is NULL");
+  if (nsteps < 0) return fail(PTF_ERR_INVALID, "nsteps must be >= 0");
+  if (!use_graph) {
+    for (int64_t i = 0; i < nsteps; ++i) CHK(ensemble_record_step(ens));
+    HIPCHK(hipGetLastError());
+    return PTF_OK;
+  }
+  if (!ens->exec) {
+    HIPCHK(hipStreamBeginCapture(ens->stream, hipStreamCaptureModeThreadLocal));
+    int rc = ensemble_record_step(ens);
+    hipGraph_t graph = nullptr;
+    hipError_t ce = hipStreamEndCapture(ens->stream, &graph);
+    if (rc != PTF_OK) {
+      if (graph) (void)hipGraphDestroy(graph);
+      return rc;
+    }
+    HIPCHK(ce);
+    ens->graph = graph;
+    HIPCHK(hipGraphInstantiate(&ens->exec, ens->graph, nullptr, nullptr, 0));
+  }
+  for (int64_t i = 0; i < nsteps; ++i) HIPCHK(hipGraphLaunch(ens->exec, ens->stream));
+  return PTF_OK;
+}
+
+int ptf_ensemble_autotune(ptf_ensemble* ens) {
+  if (!ens) return fail(PTF_ERR_INVALID, "ensemble is NULL");
+  for (auto& g : ens->groups) CHK(autotune_group(*g, ens->stream));
+  drop_graph(ens);
+  return PTF_OK;
+}
+
+static std::vector<std::pair<Group*, Slot*>> ens_slots(ptf_ensemble* e) {
+  std::vector<std::pair<Group*, Slot*>> v;
+  for (auto& g : e->groups)
+    for (Slot* s : step_slots(*g)) v.push_back({g.get(), s});
+  return v;
+}
+
+int ptf_ensemble_num_slots(ptf_ensemble* ens, int32_t* n) {
+  if (!ens || !n) return fail(PTF_ERR_INVALID, "NULL argument");
+  *n = (int32_t)ens_slots(ens).size();
+  return PTF_OK;
+}
+
+int ptf_ensemble_slot_name(ptf_ensemble* ens, int32_t slot, char* buf, int32_t buflen) {
+  if (!ens) return fail(PTF_ERR_INVALID, "ensemble is NULL");
+  auto v = ens_slots(ens);
+  if (slot < 0 || slot >= (int)v.size()) return fail(PTF_ERR_INVALID, "slot out of range");
+  return slot_name_into(*v[slot].second, buf, buflen);
+}
+
+int ptf_ensemble_time_slot(ptf_ensemble* ens, int32_t slot, int32_t reps, double* avg_ms) {
+  if (!ens || !avg_ms) return fail(PTF_ERR_INVALID, "NULL argument");
+  auto v = ens_slots(ens);
+  if (slot < 0 || slot >= (int)v.size()) return fail(PTF_ERR_INVALID, "slot out of range");
+  return time_slot(*v[slot].first, *v[slot].second, reps, ens->stream, avg_ms);
+}
+
+}  // extern "C"

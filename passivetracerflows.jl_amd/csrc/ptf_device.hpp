@@ -1,0 +1,355 @@
+// Device building blocks of libptf_hip (gfx950): complex arithmetic, radix-2/4/8/16
+// register butterflies, and a Stockham autosort line FFT whose exchanges go
+// through LDS.
+//
+// Register contract of line_fft: a line of length N is processed by P = N/R
+// threads; thread j holds elements j + q*P (q = 0..R-1) on entry AND on exit
+// ("natural" layout).  Because input and output positions of a thread
+// coincide, a kernel can read a line, transform it, and write it back in place
+// without a barrier, and an inverse FFT can feed a forward FFT with no data
+// exchange (the fused advection kernels rely on this).
+//
+// Stockham formulation (Van Loan, "Computational Frameworks for the FFT",
+// 1992, sect. 1.7): pass with radix Rp and span Ns (product of earlier radices):
+//   butterfly b reads in[b + r*N/Rp], multiplies element r by
+//   exp(DIR*2*pi*i*r*(b mod Ns)/(Ns*Rp)), does an Rp-point DFT and writes
+//   out[(b / Ns)*Ns*Rp + (b mod Ns) + r*Ns].
+// N = R0 * R^K with R0 <= R: the first pass (radix R0, Ns = 1, no twiddles)
+// works on registers, the K uniform passes follow.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace ptf {
+
+template <typename T>
+struct alignas(2 * sizeof(T)) cpx {
+  T re, im;
+};
+
+template <typename T>
+__device__ __forceinline__ cpx<T> operator+(cpx<T> a, cpx<T> b) { return {a.re + b.re, a.im + b.im}; }
+template <typename T>
+__device__ __forceinline__ cpx<T> operator-(cpx<T> a, cpx<T> b) { return {a.re - b.re, a.im - b.im}; }
+template <typename T>
+__device__ __forceinline__ cpx<T> cmul(cpx<T> a, cpx<T> b) {
+  return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+template <typename T>
+__device__ __forceinline__ cpx<T> cconj(cpx<T> a) { return {a.re, -a.im}; }
+template <typename T>
+__device__ __forceinline__ cpx<T> cscale(T s, cpx<T> a) { return {s * a.re, s * a.im}; }
+// a * (i*k) for real k: (re + i im)(i k) = (-k im) + i (k re)
+template <typename T>
+__device__ __forceinline__ cpx<T> times_ik(cpx<T> a, T k) { return {-(k * a.im), k * a.re}; }
+
+// multiply by DIR*i  (DIR = -1: forward kernel exp(-2 pi i/N); +1: inverse)
+template <int DIR, typename T>
+__device__ __forceinline__ cpx<T> mul_i(cpx<T> a) {
+  if constexpr (DIR < 0) return {a.im, -a.re};
+  else return {-a.im, a.re};
+}
+
+// exp(DIR * 2 pi i m / 16) * a, m a compile-time constant
+template <int M, int DIR, typename T>
+__device__ __forceinline__ cpx<T> rot16(cpx<T> a) {
+  constexpr int m = M & 15;
+  if constexpr (m == 0) {
+    return a;
+  } else if constexpr (m == 4) {
+    return mul_i<DIR>(a);
+  } else if constexpr (m == 8) {
+    return {-a.re, -a.im};
+  } else if constexpr (m == 12) {
+    return mul_i<-DIR>(a);
+  } else {
+    // cos and sin of 2 pi m / 16
+    constexpr double C1 = 0.92387953251128675613;  // cos(pi/8)
+    constexpr double S1 = 0.38268343236508977173;  // sin(pi/8)
+    constexpr double H = 0.70710678118654752440;   // cos(pi/4)
+    constexpr double CT[16] = {1, C1, H, S1, 0, -S1, -H, -C1, -1, -C1, -H, -S1, 0, S1, H, C1};
+    constexpr double ST[16] = {0, S1, H, C1, 1, C1, H, S1, 0, -S1, -H, -C1, -1, -C1, -H, -S1};
+    const T c = (T)CT[m];
+    const T s = (T)(DIR * ST[m]);
+    return {a.re * c - a.im * s, a.re * s + a.im * c};
+  }
+}
+
+template <int DIR, typename T>
+__device__ __forceinline__ void dft2(cpx<T>& a0, cpx<T>& a1) {
+  cpx<T> t = a0;
+  a0 = t + a1;
+  a1 = t - a1;
+}
+
+template <int DIR, typename T>
+__device__ __forceinline__ void dft4(cpx<T>& a0, cpx<T>& a1, cpx<T>& a2, cpx<T>& a3) {
+  cpx<T> t0 = a0 + a2, t1 = a0 - a2, t2 = a1 + a3, t3 = mul_i<DIR>(a1 - a3);
+  a0 = t0 + t2;
+  a2 = t0 - t2;
+  a1 = t1 + t3;
+  a3 = t1 - t3;
+}
+
+// 8-point DFT, natural order in and out: n = 2 n1 + n2, k = k1 + 4 k2
+template <int DIR, typename T>
+__device__ __forceinline__ void dft8(cpx<T>* a) {
+  dft4<DIR>(a[0], a[2], a[4], a[6]);
+  dft4<DIR>(a[1], a[3], a[5], a[7]);
+  a[3] = rot16<2, DIR>(a[3]);
+  a[5] = rot16<4, DIR>(a[5]);
+  a[7] = rot16<6, DIR>(a[7]);
+  cpx<T> o[8];
+#pragma unroll
+  for (int k1 = 0; k1 < 4; ++k1) {
+    o[k1] = a[2 * k1] + a[2 * k1 + 1];
+    o[k1 + 4] = a[2 * k1] - a[2 * k1 + 1];
+  }
+#pragma unroll
+  for (int i = 0; i < 8; ++i) a[i] = o[i];
+}
+
+// 16-point DFT, natural order in and out: n = 4 n1 + n2, k = k1 + 4 k2
+template <int DIR, typename T>
+__device__ __forceinline__ void dft16(cpx<T>* a) {
+  dft4<DIR>(a[0], a[4], a[8], a[12]);
+  dft4<DIR>(a[1], a[5], a[9], a[13]);
+  dft4<DIR>(a[2], a[6], a[10], a[14]);
+  dft4<DIR>(a[3], a[7], a[11], a[15]);
+  // a[4 k1 + n2] *= w16^(n2 k1)
+  a[5] = rot16<1, DIR>(a[5]);
+  a[6] = rot16<2, DIR>(a[6]);
+  a[7] = rot16<3, DIR>(a[7]);
+  a[9] = rot16<2, DIR>(a[9]);
+  a[10] = rot16<4, DIR>(a[10]);
+  a[11] = rot16<6, DIR>(a[11]);
+  a[13] = rot16<3, DIR>(a[13]);
+  a[14] = rot16<6, DIR>(a[14]);
+  a[15] = rot16<9, DIR>(a[15]);
+  dft4<DIR>(a[0], a[1], a[2], a[3]);
+  dft4<DIR>(a[4], a[5], a[6], a[7]);
+  dft4<DIR>(a[8], a[9], a[10], a[11]);
+  dft4<DIR>(a[12], a[13], a[14], a[15]);
+  // X[k1 + 4 k2] sits in a[4 k1 + k2]: transpose the 4x4 register block
+  cpx<T> o[16];
+#pragma unroll
+  for (int k1 = 0; k1 < 4; ++k1)
+#pragma unroll
+    for (int k2 = 0; k2 < 4; ++k2) o[k1 + 4 * k2] = a[4 * k1 + k2];
+#pragma unroll
+  for (int i = 0; i < 16; ++i) a[i] = o[i];
+}
+
+template <int R, int DIR, typename T>
+__device__ __forceinline__ void dft(cpx<T>* a) {
+  if constexpr (R == 2) dft2<DIR>(a[0], a[1]);
+  else if constexpr (R == 4) dft4<DIR>(a[0], a[1], a[2], a[3]);
+  else if constexpr (R == 8) dft8<DIR>(a);
+  else if constexpr (R == 16) dft16<DIR>(a);
+  else static_assert(R == 2, "unsupported radix");
+}
+
+constexpr int uniform_passes(int N, int R) {
+  int k = 0;
+  while (N > R && N % R == 0) {
+    N /= R;
+    ++k;
+  }
+  return k;
+}
+constexpr int ipow(int b, int e) { return e == 0 ? 1 : b * ipow(b, e - 1); }
+
+// LDS layouts: line c, element i
+template <int NL>
+struct LayInterleaved {  // lines interleaved: consecutive lanes = consecutive lines
+  static __device__ __forceinline__ int idx(int c, int i) { return i * NL + c; }
+};
+template <int N, int PAD>
+struct LayRows {  // lines contiguous: consecutive lanes = consecutive elements
+  static __device__ __forceinline__ int idx(int c, int i) { return c * (N + PAD) + i; }
+};
+
+// Line FFT of length N with radix R on P = N/R threads per line (see header).
+// tw[m] = exp(-2 pi i m / LT) with LT = N * twstride.  Starts with a barrier
+// before its first LDS write; ends after its last LDS read (callers that write
+// LDS afterwards must barrier first).  Every thread of the block must call it.
+template <typename T, int N, int R, int DIR, class Lay>
+__device__ __forceinline__ void line_fft(cpx<T> (&a)[R], cpx<T>* lds, int c, int j,
+                                         const cpx<T>* __restrict__ tw, int twstride) {
+  constexpr int P = N / R;
+  constexpr int K = uniform_passes(N, R);
+  constexpr int R0 = N / ipow(R, K);
+  static_assert(R0 * ipow(R, K) == N && R0 >= 2 && R0 <= R, "N must be R0 * R^K");
+  if constexpr (K == 0) {
+    dft<R, DIR>(a);
+    return;
+  } else {
+    constexpr int B = R / R0;
+    __syncthreads();
+#pragma unroll
+    for (int b = 0; b < B; ++b) {
+      cpx<T> t[R0];
+#pragma unroll
+      for (int r = 0; r < R0; ++r) t[r] = a[b + r * B];
+      dft<R0, DIR>(t);
+      const int jj = j + b * P;
+#pragma unroll
+      for (int r = 0; r < R0; ++r) lds[Lay::idx(c, jj * R0 + r)] = t[r];
+    }
+    int Ns = R0;
+#pragma unroll
+    for (int p = 1; p <= K; ++p) {
+      __syncthreads();
+#pragma unroll
+      for (int r = 0; r < R; ++r) a[r] = lds[Lay::idx(c, j + r * P)];
+      const int m = j % Ns;
+      const int step = (N / (Ns * R)) * twstride;
+#pragma unroll
+      for (int r = 1; r < R; ++r) {
+        cpx<T> w = tw[r * m * step];
+        if constexpr (DIR > 0) w.im = -w.im;
+        a[r] = cmul(a[r], w);
+      }
+      dft<R, DIR>(a);
+      if (p < K) {
+        __syncthreads();
+        const int base = (j / Ns) * Ns * R + m;
+#pragma unroll
+        for (int r = 0; r < R; ++r) lds[Lay::idx(c, base + r * Ns)] = a[r];
+        Ns *= R;
+      }
+    }
+  }
+}
+
+// --------------------------------------------------------------------------
+// linear operator and stage updates
+// --------------------------------------------------------------------------
+// signed wavenumber index of fftfreq (Nyquist negative)
+__device__ __forceinline__ int fftfreq_index(int i, int n) { return i < n / 2 ? i : i - n; }
+
+__device__ __forceinline__ double ipow_d(double x, int n) {
+  double r = 1.0;
+  double b = x;
+  while (n > 0) {
+    if (n & 1) r *= b;
+    b *= b;
+    n >>= 1;
+  }
+  return r;
+}
+
+struct OpParams {   // per-member linear-operator coefficients
+  double kappa, eta, iota, kappah;
+};
+
+struct GridInfo {
+  int ndim, nx, ny, nz;     // physical sizes (ny = nz = 1 below ndim)
+  double ksx, ksy, ksz;     // wavenumber scales 2 pi / L
+};
+
+// L = -kappa kr^2 - eta l^2 - iota m^2 - kappah Krsq^nkappah   (1D: (kr^2)^n)
+__device__ __forceinline__ double linear_op(const GridInfo& g, const OpParams& p, int nkappah,
+                                            int ikx, int iky, int ikz) {
+  const double kr = ikx * g.ksx;
+  const double kr2 = kr * kr;
+  if (g.ndim == 1) return -p.kappa * kr2 - p.kappah * ipow_d(kr2, nkappah);
+  const double l = fftfreq_index(iky, g.ny) * g.ksy;
+  const double l2 = l * l;
+  if (g.ndim == 2) return -p.kappa * kr2 - p.eta * l2 - p.kappah * ipow_d(kr2 + l2, nkappah);
+  const double m = fftfreq_index(ikz, g.nz) * g.ksz;
+  const double m2 = m * m;
+  return -p.kappa * kr2 - p.eta * l2 - p.iota * m2 - p.kappah * ipow_d(kr2 + l2 + m2, nkappah);
+}
+
+enum { EPI_STORE = 0, EPI_RK4 = 1, EPI_ETDRK4 = 2 };
+
+template <typename T>
+struct StageArgs {
+  int kind;     // EPI_*
+  int stage;    // 0..3
+  cpx<T>* sol;  // state
+  cpx<T>* stg;  // RK4: stage input sol1;  ETDRK4: sol1
+  cpx<T>* stg2; // ETDRK4: sol2
+  cpx<T>* acc;  // RK4: running RHS sum; ETDRK4: partial update
+  cpx<T>* n1;   // ETDRK4: N1
+  cpx<T>* n2;   // ETDRK4: N2
+  const double* E;      // exp(L dt)
+  const double* E2;     // exp(L dt / 2)
+  const double* zeta;
+  const double* alpha;
+  const double* beta;
+  const double* gam;
+  double dt;
+  const OpParams* op;   // per member
+  int nkappah;
+  GridInfo g;
+  long long mstride;    // spectral elements per member
+};
+
+// Stage update at spectral element idx (member-relative wavenumber indices
+// ikx, iky, ikz) given the nonlinear term N of the stage input.
+// RK4 ([FF] RK4substeps!/RK4update!): RHS_s = N_s + L x_s;
+//   x_2 = sol + dt/2 RHS_1, x_3 = sol + dt/2 RHS_2, x_4 = sol + dt RHS_3,
+//   sol += dt (RHS_1/6 + RHS_2/3 + RHS_3/3 + RHS_4/6)   (sum in this order)
+// ETDRK4 ([FF] ETDRK4substeps!/ETDRK4update!):
+//   sol1 = E2 sol + zeta N1 ; sol2 = E2 sol + zeta N2 ; sol2 = E2 sol1 + zeta (2 N3 - N1);
+//   sol = E sol + alpha N1 + 2 beta (N2 + N3) + Gamma N4
+template <typename T>
+__device__ __forceinline__ void stage_epilogue(const StageArgs<T>& a, long long idx, int member,
+                                               int ikx, int iky, int ikz, cpx<T> N) {
+  if (a.kind == EPI_RK4) {
+    const double Ld = linear_op(a.g, a.op[member], a.nkappah, ikx, iky, ikz);
+    const T L = (T)Ld;
+    const T dt = (T)a.dt;
+    cpx<T> x = a.stage == 0 ? a.sol[idx] : a.stg[idx];
+    cpx<T> rhs = {N.re + L * x.re, N.im + L * x.im};
+    if (a.stage == 0) {
+      const T h = (T)(a.dt / 2);
+      a.stg[idx] = {x.re + h * rhs.re, x.im + h * rhs.im};
+      a.acc[idx] = {rhs.re / (T)6, rhs.im / (T)6};
+    } else if (a.stage == 1 || a.stage == 2) {
+      const T h = a.stage == 1 ? (T)(a.dt / 2) : dt;
+      cpx<T> s = a.sol[idx];
+      cpx<T> ac = a.acc[idx];
+      a.stg[idx] = {s.re + h * rhs.re, s.im + h * rhs.im};
+      a.acc[idx] = {ac.re + rhs.re / (T)3, ac.im + rhs.im / (T)3};
+    } else {
+      cpx<T> s = a.sol[idx];
+      cpx<T> ac = a.acc[idx];
+      cpx<T> tot = {ac.re + rhs.re / (T)6, ac.im + rhs.im / (T)6};
+      a.sol[idx] = {s.re + dt * tot.re, s.im + dt * tot.im};
+    }
+  } else if (a.kind == EPI_ETDRK4) {
+    // coefficients are FP64 arrays; the update runs in FP64 and rounds once
+    const double nre = (double)N.re, nim = (double)N.im;
+    if (a.stage == 0) {
+      const cpx<T> s = a.sol[idx];
+      const double e2 = a.E2[idx], z = a.zeta[idx];
+      a.stg[idx] = {(T)(e2 * (double)s.re + z * nre), (T)(e2 * (double)s.im + z * nim)};
+      a.n1[idx] = N;
+    } else if (a.stage == 1) {
+      const cpx<T> s = a.sol[idx];
+      const double e2 = a.E2[idx], z = a.zeta[idx];
+      a.stg2[idx] = {(T)(e2 * (double)s.re + z * nre), (T)(e2 * (double)s.im + z * nim)};
+      a.n2[idx] = N;
+    } else if (a.stage == 2) {
+      const cpx<T> s1 = a.stg[idx];
+      const cpx<T> s = a.sol[idx];
+      const cpx<T> N1 = a.n1[idx];
+      const cpx<T> N2 = a.n2[idx];
+      const double e2 = a.E2[idx], z = a.zeta[idx];
+      const double e = a.E[idx], al = a.alpha[idx], be2 = 2.0 * a.beta[idx];
+      a.stg2[idx] = {(T)(e2 * (double)s1.re + z * (2.0 * nre - (double)N1.re)),
+                     (T)(e2 * (double)s1.im + z * (2.0 * nim - (double)N1.im))};
+      a.acc[idx] = {(T)(e * (double)s.re + al * (double)N1.re + be2 * ((double)N2.re + nre)),
+                    (T)(e * (double)s.im + al * (double)N1.im + be2 * ((double)N2.im + nim))};
+    } else {
+      const cpx<T> ac = a.acc[idx];
+      const double g = a.gam[idx];
+      a.sol[idx] = {(T)((double)ac.re + g * nre), (T)((double)ac.im + g * nim)};
+    }
+  }
+}
+
+}  // namespace ptf

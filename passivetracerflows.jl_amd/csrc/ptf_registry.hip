@@ -1,0 +1,131 @@
+// Kernel-variant registry.  This file is compiled once per (precision, kind)
+// pair, with -DPTF_T=<double|float> -DPTF_TNAME=<f64|f32> -DPTF_KIND=<k>, and
+// exports  const void* ptf_reg_<TNAME>_<KIND>(int N, int R, int W)  returning
+// the kernel for line length N (column kinds: axis length; row kinds: nx),
+// radix R and width W (columns per block TX, or rows per block NL), or NULL
+// when that variant does not exist.  Only variants that satisfy the launch
+// and LDS limits are instantiated.
+#include "ptf_kernels.hpp"
+
+using namespace ptf;
+
+#ifndef PTF_T
+#error "PTF_T must be defined"
+#endif
+
+namespace {
+
+// kinds
+constexpr int K_COL_FWD = 0, K_COL_INV = 1, K_COL_D1 = 2, K_COL_D2 = 3, K_COL_STAGE = 4,
+              K_COL_FLOWSYN = 5;
+constexpr int K_ROW_C2R = 10, K_ROW_R2C = 11, K_ROW_ADV1 = 12, K_ROW_ADV2 = 13, K_ROW_ADV3 = 14;
+
+constexpr int col_mode(int k) {
+  return k == K_COL_FWD || k == K_COL_INV ? COL_PLAIN
+         : k == K_COL_D1                  ? COL_D1
+         : k == K_COL_D2                  ? COL_D2
+         : k == K_COL_STAGE               ? COL_STAGE
+                                          : COL_FLOWSYN;
+}
+constexpr int col_dir(int k) { return (k == K_COL_FWD || k == K_COL_STAGE) ? -1 : +1; }
+
+constexpr int row_mode(int k) { return k == K_ROW_C2R ? ROW_C2R : k == K_ROW_R2C ? ROW_R2C : ROW_ADV; }
+constexpr int row_nin(int k) { return k == K_ROW_ADV2 ? 2 : k == K_ROW_ADV3 ? 3 : 1; }
+
+template <typename T, int N, int R, int TX, int KIND>
+const void* colp() {
+  constexpr int P = N / R;
+  constexpr bool ok = R <= N && (R >= 8 || N <= 16) && TX * P >= 16 && TX * P <= 512 &&
+                      (long long)N * TX * (long long)sizeof(cpx<T>) <= 65536;
+  if constexpr (ok) {
+    return reinterpret_cast<const void*>(&col_kernel<T, N, R, TX, col_mode(KIND), col_dir(KIND)>);
+  } else {
+    return nullptr;
+  }
+}
+
+template <typename T, int NX, int R, int NL, int KIND>
+const void* rowp() {
+  constexpr int M = NX / 2;
+  constexpr int P = M / R;
+  constexpr bool ok = R <= M && (R >= 8 || M <= 16) && NL * P >= 16 && NL * P <= 512 &&
+                      (long long)NL * (M + 1) * (long long)sizeof(cpx<T>) <= 65536;
+  if constexpr (ok) {
+    return reinterpret_cast<const void*>(&row_kernel<T, NX, R, NL, row_mode(KIND), row_nin(KIND)>);
+  } else {
+    return nullptr;
+  }
+}
+
+template <typename T, int N, int R, int KIND>
+const void* by_w(int W) {
+  if constexpr (KIND < 10) {
+    switch (W) {
+      case 2: return colp<T, N, R, 2, KIND>();
+      case 4: return colp<T, N, R, 4, KIND>();
+      case 8: return colp<T, N, R, 8, KIND>();
+      case 16: return colp<T, N, R, 16, KIND>();
+      case 32: return colp<T, N, R, 32, KIND>();
+      default: return nullptr;
+    }
+  } else {
+    switch (W) {
+      case 1: return rowp<T, N, R, 1, KIND>();
+      case 2: return rowp<T, N, R, 2, KIND>();
+      case 4: return rowp<T, N, R, 4, KIND>();
+      case 8: return rowp<T, N, R, 8, KIND>();
+      case 16: return rowp<T, N, R, 16, KIND>();
+      default: return nullptr;
+    }
+  }
+}
+
+template <typename T, int N, int KIND>
+const void* by_r(int R, int W) {
+  switch (R) {
+    case 4: return by_w<T, N, 4, KIND>(W);
+    case 8: return by_w<T, N, 8, KIND>(W);
+    case 16: return by_w<T, N, 16, KIND>(W);
+    default: return nullptr;
+  }
+}
+
+template <typename T, int KIND>
+const void* by_n(int N, int R, int W) {
+  if constexpr (KIND < 10) {
+    switch (N) {
+      case 8: return by_r<T, 8, KIND>(R, W);
+      case 16: return by_r<T, 16, KIND>(R, W);
+      case 32: return by_r<T, 32, KIND>(R, W);
+      case 64: return by_r<T, 64, KIND>(R, W);
+      case 128: return by_r<T, 128, KIND>(R, W);
+      case 256: return by_r<T, 256, KIND>(R, W);
+      case 512: return by_r<T, 512, KIND>(R, W);
+      case 1024: return by_r<T, 1024, KIND>(R, W);
+      case 2048: return by_r<T, 2048, KIND>(R, W);
+      default: return nullptr;
+    }
+  } else {
+    switch (N) {  // nx
+      case 16: return by_r<T, 16, KIND>(R, W);
+      case 32: return by_r<T, 32, KIND>(R, W);
+      case 64: return by_r<T, 64, KIND>(R, W);
+      case 128: return by_r<T, 128, KIND>(R, W);
+      case 256: return by_r<T, 256, KIND>(R, W);
+      case 512: return by_r<T, 512, KIND>(R, W);
+      case 1024: return by_r<T, 1024, KIND>(R, W);
+      case 2048: return by_r<T, 2048, KIND>(R, W);
+      case 4096: return by_r<T, 4096, KIND>(R, W);
+      default: return nullptr;
+    }
+  }
+}
+
+}  // namespace
+
+#define PTF_JOIN3_(a, b, c) a##_##b##_##c
+#define PTF_JOIN3(a, b, c) PTF_JOIN3_(a, b, c)
+
+extern "C" const void* PTF_JOIN3(ptf_reg, PTF_TNAME, PTF_KIND)(int N, int R, int W) {
+  return by_n<PTF_T, PTF_KIND>(N, R, W);
+}

@@ -1,0 +1,124 @@
+// Setup, conversion and diagnostic kernels of libptf_hip (host runtime only).
+#pragma once
+#include "ptf_kernels.hpp"
+
+namespace ptf {
+
+// --------------------------------------------------------------------------
+// setup, conversion and diagnostic kernels
+// --------------------------------------------------------------------------
+
+// ETDRK4 coefficients ([FF] getetdcoeffs(dt, L; ncirc=32, rcirc=1), getexpLs): FP64
+__global__ void etd_coeff_kernel(const double* __restrict__ L, long long n, double dt, int ncirc,
+                                 double rcirc, double* E, double* E2, double* zeta, double* alpha,
+                                 double* beta, double* gam) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double zr = dt * L[i];
+  double sz = 0, sa = 0, sb = 0, sg = 0;  // real parts of the contour sums
+  for (int jj = 0; jj < ncirc; ++jj) {
+    double sn, cs;
+    sincospi(2.0 * (jj + 0.5) / ncirc, &sn, &cs);
+    // z = zr + rcirc e^{i theta}
+    const double xr = zr + rcirc * cs, xi = rcirc * sn;
+    // e^z, e^{z/2}
+    const double ea = exp(xr);
+    double s1, c1;
+    sincos(xi, &s1, &c1);
+    const double ezr = ea * c1, ezi = ea * s1;
+    const double eh = exp(xr / 2);
+    double s2, c2;
+    sincos(xi / 2, &s2, &c2);
+    const double ehr = eh * c2, ehi = eh * s2;
+    // 1/z, 1/z^3
+    const double den = xr * xr + xi * xi;
+    const double izr = xr / den, izi = -xi / den;
+    const double z2r = xr * xr - xi * xi, z2i = 2 * xr * xi;
+    const double iz2r = izr * izr - izi * izi, iz2i = 2 * izr * izi;
+    const double iz3r = iz2r * izr - iz2i * izi, iz3i = iz2r * izi + iz2i * izr;
+    // zeta: (e^{z/2} - 1)/z
+    {
+      const double nr = ehr - 1, ni = ehi;
+      sz += nr * izr - ni * izi;
+    }
+    // alpha: (-4 - z + e^z (4 - 3z + z^2))/z^3
+    {
+      const double pr = 4 - 3 * xr + z2r, pi = -3 * xi + z2i;
+      const double nr = -4 - xr + (ezr * pr - ezi * pi);
+      const double ni = -xi + (ezr * pi + ezi * pr);
+      sa += nr * iz3r - ni * iz3i;
+    }
+    // beta: (2 + z + e^z (-2 + z))/z^3
+    {
+      const double pr = -2 + xr, pi = xi;
+      const double nr = 2 + xr + (ezr * pr - ezi * pi);
+      const double ni = xi + (ezr * pi + ezi * pr);
+      sb += nr * iz3r - ni * iz3i;
+    }
+    // Gamma: (-4 - 3z - z^2 + e^z (4 - z))/z^3
+    {
+      const double pr = 4 - xr, pi = -xi;
+      const double nr = -4 - 3 * xr - z2r + (ezr * pr - ezi * pi);
+      const double ni = -3 * xi - z2i + (ezr * pi + ezi * pr);
+      sg += nr * iz3r - ni * iz3i;
+    }
+  }
+  zeta[i] = dt * (sz / ncirc);
+  alpha[i] = dt * (sa / ncirc);
+  beta[i] = dt * (sb / ncirc);
+  gam[i] = dt * (sg / ncirc);
+  E[i] = exp(dt * L[i]);
+  E2[i] = exp(dt * L[i] / 2);
+}
+
+template <typename TO, typename TI>
+__global__ void convert_kernel(TO* __restrict__ out, const TI* __restrict__ in, long long n) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = (TO)in[i];
+}
+
+template <typename T>
+__global__ void copy_kernel(T* __restrict__ out, const T* __restrict__ in, long long n) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = in[i];
+}
+
+// Neumaier-compensated FP64 partial sums of w_k |sol_k|^2 (Hermitian weights
+// 1 at kx = 0 and nx/2, 2 elsewhere), one (sum, compensation) pair per block.
+template <typename T>
+__global__ void variance_partial_kernel(const cpx<T>* __restrict__ sol, long long n, int nxh,
+                                        double* __restrict__ partial) {
+  __shared__ double ssum[256];
+  __shared__ double scomp[256];
+  double sum = 0.0, comp = 0.0;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const int kx = (int)(i % nxh);
+    const double w = (kx == 0 || kx == nxh - 1) ? 1.0 : 2.0;
+    const cpx<T> v = sol[i];
+    const double x = w * ((double)v.re * (double)v.re + (double)v.im * (double)v.im);
+    const double tt = sum + x;
+    if (fabs(sum) >= fabs(x)) comp += (sum - tt) + x;
+    else comp += (x - tt) + sum;
+    sum = tt;
+  }
+  ssum[threadIdx.x] = sum;
+  scomp[threadIdx.x] = comp;
+  __syncthreads();
+  for (int off = blockDim.x / 2; off > 0; off >>= 1) {
+    if ((int)threadIdx.x < off) {
+      const double a = ssum[threadIdx.x], b = ssum[threadIdx.x + off];
+      const double tt = a + b;
+      double e = (fabs(a) >= fabs(b)) ? (a - tt) + b : (b - tt) + a;
+      ssum[threadIdx.x] = tt;
+      scomp[threadIdx.x] += scomp[threadIdx.x + off] + e;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    partial[2 * blockIdx.x] = ssum[0];
+    partial[2 * blockIdx.x + 1] = scomp[0];
+  }
+}
+
+}  // namespace ptf

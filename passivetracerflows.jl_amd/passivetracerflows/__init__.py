@@ -1,0 +1,24 @@
+"""passivetracerflows: the PassiveTracerFlows.jl TracerAdvectionDiffusion time step on MI355X.
+
+The physics API mirrors the Julia package (``TracerAdvectionDiffusion`` module,
+FourierFlows ``Problem``/``stepforward!``/grids) with Julia's ``!`` written as
+a trailing ``_``.  Every transform and time step runs in the HIP library
+libptf_hip (gfx950); this package is the host-side binding.
+"""
+from . import _lib
+from . import fourierflows as FourierFlows
+from . import traceradvectiondiffusion as TracerAdvectionDiffusion
+from .ensemble import Ensemble
+from .fourierflows import (GPU, Clock, DeviceArray, OneDGrid, ROCm, ThreeDGrid, TwoDGrid, gridpoints,
+                           ldiv_, mul_, step_until_, stepforward_)
+from .traceradvectiondiffusion import (OneDAdvectingFlow, SpectralStreamfunctionFlow, ThreeDAdvectingFlow,
+                                       TwoDAdvectingFlow, set_c_, updatevars_)
+
+stepforward = stepforward_
+
+__all__ = [
+    "FourierFlows", "TracerAdvectionDiffusion", "Ensemble", "ROCm", "GPU", "Clock", "DeviceArray",
+    "OneDGrid", "TwoDGrid", "ThreeDGrid", "gridpoints", "mul_", "ldiv_", "stepforward_", "step_until_",
+    "OneDAdvectingFlow", "TwoDAdvectingFlow", "ThreeDAdvectingFlow", "SpectralStreamfunctionFlow",
+    "set_c_", "updatevars_",
+]

@@ -1,0 +1,99 @@
+"""Ensembles of independent 2D tracer problems advanced by one graph-captured launch sequence.
+
+Each member is a ``TracerAdvectionDiffusion.Problem(dev, TwoDAdvectingFlow(u, v);
+nx, κ, η, dt, stepper="RK4")`` with a steady flow.  Members with the same grid
+share one batched set of kernels; the per-step launches of all grid sizes are
+captured once into a hipGraph (one branch per size) and replayed.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import call, dptr
+
+
+class Ensemble:
+    def __init__(self, members, dt, precision="mixed", stepper="RK4"):
+        """members: iterable of dicts with n (grid size, nx = ny), kappa, optional eta, L."""
+        members = list(members)
+        arr = (_lib.ptf_desc * len(members))()
+        self.sizes = []
+        for i, m in enumerate(members):
+            d = arr[i]
+            n = int(m["n"])
+            L = float(m.get("L", 2 * np.pi))
+            d.ndim = 2
+            d.precision = _lib.PTF_MIXED if precision == "mixed" else _lib.PTF_FP64
+            d.n[0], d.n[1], d.n[2] = n, n, 1
+            d.L[0], d.L[1], d.L[2] = L, L, 1.0
+            d.kappa = float(m["kappa"])
+            d.eta = float(m.get("eta", m["kappa"]))
+            d.iota = d.kappa
+            d.kappah = float(m.get("kappah", 0.0))
+            d.nkappah = int(m.get("nkappah", 0))
+            d.stepper = _lib.PTF_RK4 if stepper.upper() == "RK4" else _lib.PTF_ETDRK4
+            d.dt = float(dt)
+            d.flow_kind = _lib.PTF_FLOW_STEADY
+            self.sizes.append(n)
+        h = C.c_void_p()
+        call("ptf_ensemble_create", arr, len(members), C.byref(h))
+        self._h = h.value
+        self.count = len(members)
+        self.steps = 0
+
+    def set_flow(self, i, u, v):
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        call("ptf_ensemble_set_flow", self._h, i, dptr(u), dptr(v))
+
+    def set_c(self, i, c):
+        c = np.ascontiguousarray(c, dtype=np.float64)
+        call("ptf_ensemble_set_c", self._h, i, dptr(c))
+
+    def get_c(self, i):
+        n = self.sizes[i]
+        out = np.empty((n, n))
+        call("ptf_ensemble_get_c", self._h, i, dptr(out))
+        return out
+
+    def get_sol(self, i):
+        n = self.sizes[i]
+        out = np.empty((n, n // 2 + 1), np.complex128)
+        call("ptf_ensemble_get_sol", self._h, i, dptr(out.view(np.float64)))
+        return out
+
+    def step(self, nsteps=1, graph=True):
+        call("ptf_ensemble_step", self._h, int(nsteps), 1 if graph else 0)
+        self.steps += int(nsteps)
+
+    def set_stream(self, hip_stream):
+        call("ptf_ensemble_set_stream", self._h, hip_stream)
+
+    def autotune(self):
+        call("ptf_ensemble_autotune", self._h)
+
+    def slots(self):
+        n = C.c_int32()
+        call("ptf_ensemble_num_slots", self._h, C.byref(n))
+        out = []
+        for i in range(n.value):
+            buf = C.create_string_buffer(256)
+            call("ptf_ensemble_slot_name", self._h, i, buf, 256)
+            out.append(buf.value.decode())
+        return out
+
+    def time_slot(self, i, reps=20):
+        ms = C.c_double()
+        call("ptf_ensemble_time_slot", self._h, i, reps, C.byref(ms))
+        return ms.value
+
+    def __del__(self):
+        try:
+            if getattr(self, "_h", None):
+                _lib.load().ptf_ensemble_destroy(self._h)
+                self._h = None
+        except Exception:
+            pass
