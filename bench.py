@@ -1,0 +1,298 @@
+"""Benchmark: TracerAdvectionDiffusion time step on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-extra]
+
+Headline workload: 3D steady ABC flow, 256^3, RK4, dt = 5e-3, kappa = 1e-3,
+FP64 plain form (inputs resident in HBM before the timed region).  A "step"
+is one full RK4 step (4 calcN! evaluations + stage updates) of that problem;
+value = grid points x steps / wall seconds.
+
+Also reported (``forms``): the same problem in the mixed FP32 form, the
+512-member FP32 ensemble (64^2..512^2, hipGraph replay), the 2D 256^2
+cellular flow (FP64, mixed) and the 2D 2048^2 random-phase ETDRK4 flow.
+The ``roofline`` object is for the kernel that takes the most time per step
+of the headline workload; its bytes come from the pass model of each kernel
+(see the KERNEL_BYTES table).  ``cpu_baseline`` times the FP64 oracle
+restatement (scipy pocketfft threads + numpy) on a bounded number of steps.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for p in (ROOT, os.path.join(ROOT, "passivetracerflows.jl_amd")):
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+METRIC = "grid-point-steps/s, FP64 plain vs FP32 batched at relL2≤1e-5; %HBM roofline"
+HBM_PEAK_GBS = 8000.0
+
+
+def kernel_bytes(name: str, nS: int, nR: int, cb: int, rb: int, stage: int):
+    """Algorithmic HBM bytes of one launch (pass model): every field a kernel must
+    read or write once.  cb/rb = bytes per complex/real element."""
+    S, R = nS * cb, nR * rb
+    fam = name.split(" [")[0].split("_", 1)[1] if name.startswith("stage") else name.split(" [")[0]
+    if fam == "z_inverse_deriv":
+        return 3 * S                     # read c, write IFFT_z c, IFFT_z(i m c)
+    if fam == "y_inverse_deriv2":
+        return 5 * S                     # read 2, write 3
+    if fam == "y_inverse_deriv":
+        return 3 * S                     # 2D: read 1, write 2
+    if fam == "x_advect":
+        nin = 3 if "advect3" in name else (2 if "advect2" in name else 1)
+        return (nin + 1) * S + nin * R   # read nin spectral rows + nin flows, write 1
+    if fam == "x_advect_stage":          # 1D
+        return 2 * S + R + (2 * S if stage == 0 else 3 * S)
+    if fam == "y_forward":
+        return 2 * S
+    if fam in ("z_forward_stage", "y_forward_stage"):
+        # read F; RK4 stage 0: read sol, write stg+acc; 1,2: read sol,stg,acc, write stg,acc;
+        # 3: read stg,acc,sol, write sol
+        return S + {0: 3 * S, 1: 5 * S, 2: 5 * S, 3: 4 * S}[stage]
+    return None
+
+
+def gps(points, steps, seconds):
+    return points * steps / seconds
+
+
+def sync():
+    import torch
+    torch.cuda.synchronize()
+
+
+def time_steps(stepfn, steps):
+    sync()
+    t0 = time.perf_counter()
+    stepfn(steps)
+    sync()
+    return time.perf_counter() - t0
+
+
+def headline(dev, steps, warmup, T=np.float64, autotune=True):
+    import passivetracerflows as ptf
+    from passivetracerflows import workloads as W
+    TAD = ptf.TracerAdvectionDiffusion
+    cfg = W.abc_3d(256)
+    u, v, w = cfg["flow"]
+    prob = TAD.Problem(dev, ptf.ThreeDAdvectingFlow(u=lambda x, y, z: u, v=lambda x, y, z: v,
+                                                     w=lambda x, y, z: w),
+                       nx=256, κ=cfg["kappa"], dt=cfg["dt"], stepper="RK4", T=T)
+    TAD.set_c_(prob, cfg["c0"])
+    if autotune:
+        prob.autotune()
+    m0 = TAD.mass(prob)
+    v0 = TAD.variance(prob)
+    ptf.stepforward_(prob, warmup)
+    secs = time_steps(lambda k: ptf.stepforward_(prob, k), steps)
+    m1 = TAD.mass(prob)
+    v1 = TAD.variance(prob)
+    return cfg, prob, secs, (m0, m1, v0, v1)
+
+
+def slot_table(prob, nS, nR, cb, rb, reps=20):
+    rows = []
+    for i, name in enumerate(prob.slots()):
+        ms = prob.time_slot(i, reps)
+        stage = int(name[5]) - 1 if name.startswith("stage") else 0
+        b = kernel_bytes(name, nS, nR, cb, rb, stage)
+        rows.append(dict(slot=i, name=name, ms=ms, bytes=b,
+                         gbs=(b / (ms * 1e-3) / 1e9) if b else None))
+    return rows
+
+
+def dominant(rows):
+    fams = {}
+    for r in rows:
+        key = r["name"].split(" [")[0]
+        key = key.split("_", 1)[1] if key.startswith("stage") else key
+        f = fams.setdefault(key, dict(total_ms=0.0, n=0, bytes=0, kernel=r["name"].split(" [")[1][:-1]))
+        f["total_ms"] += r["ms"]
+        f["n"] += 1
+        f["bytes"] += r["bytes"] or 0
+    name = max(fams, key=lambda k: fams[k]["total_ms"])
+    f = fams[name]
+    avg_ms = f["total_ms"] / f["n"]
+    avg_bytes = f["bytes"] / f["n"]
+    return name, f, avg_ms, avg_bytes, fams
+
+
+def cpu_baseline(cfg, steps):
+    from oracle import tad as O
+    cores = min(16, os.cpu_count() or 1)
+    O.set_workers(cores)
+    g = O.Grid(3, 256)
+    p = O.Problem(g, cfg["flow"], kappa=cfg["kappa"], dt=cfg["dt"], stepper="RK4")
+    p.set_c(cfg["c0"])
+    p.stepforward(1)  # warm-up
+    t0 = time.perf_counter()
+    p.stepforward(steps)
+    dt = time.perf_counter() - t0
+    return dict(value=gps(g.npoints, steps, dt), unit="grid-point-steps/s", cores=cores,
+                kind="port",
+                sample=f"FP64 oracle restatement (scipy.fft pocketfft, workers={cores}; numpy "
+                       f"elementwise single-threaded), 3D ABC 256^3 RK4, {steps} steps after 1 "
+                       f"warm-up step, {dt:.1f} s")
+
+
+def extra_forms(dev, steps):
+    import passivetracerflows as ptf
+    from passivetracerflows import workloads as W
+    TAD = ptf.TracerAdvectionDiffusion
+    out = {}
+    # 2D cellular 256^2, FP64 and mixed
+    cfg = W.cellular_2d(256)
+    u, v = cfg["flow"]
+    for T, tag in ((np.float64, "fp64"), (np.float32, "mixed")):
+        p = TAD.Problem(dev, ptf.TwoDAdvectingFlow(u=lambda x, y: u, v=lambda x, y: v), nx=256,
+                        κ=cfg["kappa"], dt=cfg["dt"], T=T)
+        TAD.set_c_(p, cfg["c0"])
+        p.autotune()
+        ptf.stepforward_(p, 10)
+        k = max(steps * 4, 200)
+        s = time_steps(lambda n: ptf.stepforward_(p, n), k)
+        out[f"cellular2d_256_{tag}"] = dict(value=gps(256 * 256, k, s), ms_per_step=1e3 * s / k,
+                                            steps=k)
+    # 2D 2048^2 random-phase flow, ETDRK4
+    cfg = W.turbulent_2d(2048)
+    for T, tag in ((np.float64, "fp64"), (np.float32, "mixed")):
+        p = TAD.Problem(dev, ptf.SpectralStreamfunctionFlow(cfg["psih0"], cfg["omega"]), nx=2048,
+                        κ=0.0, η=0.0, κh=cfg["kappah"], nκh=cfg["nkappah"], dt=cfg["dt"],
+                        stepper="ETDRK4", T=T)
+        TAD.set_c_(p, cfg["c0"])
+        p.autotune()
+        ptf.stepforward_(p, 3)
+        k = max(steps // 2, 20)
+        s = time_steps(lambda n: ptf.stepforward_(p, n), k)
+        out[f"turbulent2d_2048_etdrk4_{tag}"] = dict(value=gps(2048 * 2048, k, s),
+                                                     ms_per_step=1e3 * s / k, steps=k)
+    # 512-member ensemble, FP32 bulk, hipGraph; parity vs the FP64 ensemble on the GPU
+    ens_cfg = W.ensemble_2d()
+    members = [dict(m, **W.ensemble_member(m["n"], m["seed"])) for m in ens_cfg["members"]]
+    npts = sum(m["n"] ** 2 for m in members)
+    res = {}
+    for prec in ("mixed", "fp64"):
+        e = ptf.Ensemble([dict(n=m["n"], kappa=m["kappa"]) for m in members], dt=ens_cfg["dt"],
+                         precision=prec)
+        for i, m in enumerate(members):
+            e.set_flow(i, *m["flow"])
+            e.set_c(i, m["c0"])
+        if prec == "mixed":
+            e.autotune()
+        e.step(2)
+        k = max(steps, 50)
+        s = time_steps(lambda n: e.step(n, graph=True), k)
+        res[prec] = (e, s, k)
+    e32, s32, k = res["mixed"]
+    e64, s64, _ = res["fp64"]
+    worst = 0.0
+    for i in range(len(members)):
+        a, b = e32.get_sol(i), e64.get_sol(i)
+        worst = max(worst, float(np.linalg.norm(a - b) / np.linalg.norm(b)))
+    out["ensemble512_fp32_graph"] = dict(value=gps(npts, k, s32), ms_per_step=1e3 * s32 / k, steps=k,
+                                         points_per_step=npts,
+                                         max_member_relL2_vs_fp64=worst)
+    out["ensemble512_fp64_graph"] = dict(value=gps(npts, k, s64), ms_per_step=1e3 * s64 / k, steps=k)
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extra", action="store_true")
+    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--no-autotune", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: this build runs a single problem (or ensemble) per GPU "
+              "and has no multi-GPU path; use --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    import torch  # noqa: F401  (device plumbing: synchronize)
+    import passivetracerflows as ptf
+    dev = ptf.ROCm(0)
+
+    cfg, prob, secs, (m0, m1, v0, v1) = headline(dev, args.steps, args.warmup,
+                                                 autotune=not args.no_autotune)
+    npts = 256 ** 3
+    value = gps(npts, args.steps, secs)
+    nS, nR = 129 * 256 * 256, npts
+    rows = slot_table(prob, nS, nR, 16, 8)
+    dname, fam, avg_ms, avg_bytes, fams = dominant(rows)
+    achieved = avg_bytes / (avg_ms * 1e-3) / 1e9
+    step_bytes = sum(r["bytes"] for r in rows if r["bytes"])
+    step_ms_kernels = sum(r["ms"] for r in rows)
+
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic_abc3d_256_fp64.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            tj = json.load(f)
+        traffic = tj.get("per_launch_bytes", {}).get(dname)
+
+    # mixed form of the same workload + its error against the FP64 run
+    _, pm, secs32, _ = headline(dev, args.steps, args.warmup, T=np.float32,
+                                autotune=not args.no_autotune)
+    s64, s32 = prob.sol, pm.sol
+    mixed_err = float(np.linalg.norm(s32 - s64) / np.linalg.norm(s64))
+
+    forms = {
+        "abc3d_256_fp64": dict(value=value, ms_per_step=1e3 * secs / args.steps, steps=args.steps,
+                               kernel_ms_per_step=step_ms_kernels,
+                               pass_model_bytes_per_step=step_bytes,
+                               pass_model_GBs=step_bytes / (secs / args.steps) / 1e9,
+                               mass_rel_drift=abs(m1 - m0) / max(abs(m0), 1e-300),
+                               variance_ratio=v1 / v0),
+        "abc3d_256_mixed": dict(value=gps(npts, args.steps, secs32),
+                                ms_per_step=1e3 * secs32 / args.steps,
+                                relL2_vs_fp64_after_steps=mixed_err,
+                                steps_compared=args.steps + args.warmup),
+    }
+    if not args.no_extra:
+        forms.update(extra_forms(dev, args.steps))
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(cfg, args.cpu_steps)
+
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "grid-point-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * secs / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded ABC flow and random-phase k^-2 initial field)",
+        "config": {"workload": "3D steady ABC flow, nx=ny=nz=256, L=2pi, kappa=1e-3, RK4, "
+                               "dt=5e-3, FP64 plain form", "grid": [256, 256, 256],
+                   "stepper": "RK4", "points": npts},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": f"{dname} [{fam['kernel']}]", "avg_launch_ms": avg_ms,
+                     "algorithmic_bytes_per_launch": avg_bytes},
+        "cpu_baseline": cpu,
+        "kernels": {k: dict(ms_per_launch=f["total_ms"] / f["n"], launches_per_step=f["n"],
+                            GBs=(f["bytes"] / f["total_ms"] * 1e3 / 1e9) if f["bytes"] else None)
+                    for k, f in fams.items()},
+        "forms": forms,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

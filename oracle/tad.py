@@ -44,7 +44,7 @@ import os
 import numpy as np
 import scipy.fft as sfft
 
-_WORKERS = int(os.environ.get("PTF_ORACLE_WORKERS", "1"))
+_WORKERS = int(os.environ.get("PTF_ORACLE_WORKERS", str(min(16, os.cpu_count() or 1))))
 
 
 def set_workers(n: int) -> None:
