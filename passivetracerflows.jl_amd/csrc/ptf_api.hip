@@ -253,7 +253,7 @@ struct Group {
   double Ls[3] = {0, 0, 0}, ks[3] = {0, 0, 0};
   std::vector<OpParams> ops;
 
-  DevBuf d_ops, sol, stg, stg2, acc, n1, n2, w[3], flow[3], d_L, etd[6], tw[3], psih0, omega;
+  DevBuf d_ops, sol, sol_lo, stg, stg2, acc, n1, n2, w[3], flow[3], d_L, etd[6], tw[3], psih0, omega;
   DevBuf phys, spec, red, backup;  // set_c/updatevars/calcN staging, reductions, timing backup
   std::vector<Slot> stages[4];
   std::vector<Slot> flowsyn;
@@ -370,6 +370,7 @@ static StageArgs<T> stage_args(const Group& g, int stage) {
   st.kind = g.stepper == PTF_RK4 ? EPI_RK4 : EPI_ETDRK4;
   st.stage = stage;
   st.sol = (cpx<T>*)g.sol.p;
+  st.sol_lo = (cpx<T>*)g.sol_lo.p;  // NULL in the FP64 form
   st.stg = (cpx<T>*)g.stg.p;
   st.stg2 = (cpx<T>*)g.stg2.p;
   st.acc = (cpx<T>*)g.acc.p;
@@ -622,6 +623,10 @@ static int group_init(Group& g, const ptf_desc* descs, int count) {
   CHK(dalloc(g.d_ops, sizeof(OpParams) * count));
   HIPCHK(hipMemcpy(g.d_ops.p, g.ops.data(), sizeof(OpParams) * count, hipMemcpyHostToDevice));
   CHK(dalloc(g.sol, S));
+  if (!g.dbl) {
+    CHK(dalloc(g.sol_lo, S));
+    HIPCHK(hipMemset(g.sol_lo.p, 0, S));
+  }
   CHK(dalloc(g.stg, S));
   CHK(dalloc(g.acc, S));
   if (g.stepper == PTF_ETDRK4) {
@@ -802,8 +807,10 @@ static int time_slot(Group& g, Slot& s, int reps, hipStream_t st, double* avg_ms
   if (reps < 1) reps = 1;
   const size_t S = (size_t)g.nS * g.B * 2 * g.esz;
   if (s.writes_state) {
-    if (!g.backup.p) CHK(dalloc(g.backup, S));
+    if (!g.backup.p) CHK(dalloc(g.backup, 2 * S));
     HIPCHK(hipMemcpyAsync(g.backup.p, g.sol.p, S, hipMemcpyDeviceToDevice, st));
+    if (g.sol_lo.p)
+      HIPCHK(hipMemcpyAsync((char*)g.backup.p + S, g.sol_lo.p, S, hipMemcpyDeviceToDevice, st));
   }
   CHK(launch(s, st));  // warm-up
   hipEvent_t e0, e1;
@@ -819,6 +826,8 @@ static int time_slot(Group& g, Slot& s, int reps, hipStream_t st, double* avg_ms
   (void)hipEventDestroy(e1);
   if (s.writes_state) {
     HIPCHK(hipMemcpyAsync(g.sol.p, g.backup.p, S, hipMemcpyDeviceToDevice, st));
+    if (g.sol_lo.p)
+      HIPCHK(hipMemcpyAsync(g.sol_lo.p, (char*)g.backup.p + S, S, hipMemcpyDeviceToDevice, st));
     HIPCHK(hipStreamSynchronize(st));
   }
   *avg_ms = (double)ms / reps;
@@ -901,6 +910,55 @@ struct ptf_ensemble {
     if (own_stream && stream) (void)hipStreamDestroy(stream);
   }
 };
+
+// sol of one member as FP64 (hi + lo in the mixed form)
+static int get_member_sol(Group& g, int member, double* out, hipStream_t st) {
+  const size_t n = (size_t)g.nS * 2;
+  if (g.dbl) {
+    HIPCHK(hipMemcpyAsync(out, (const char*)g.sol.p + (size_t)member * n * 8, n * 8,
+                          hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return PTF_OK;
+  }
+  std::vector<float> hi(n), lo(n);
+  HIPCHK(hipMemcpyAsync(hi.data(), (const char*)g.sol.p + (size_t)member * n * 4, n * 4,
+                        hipMemcpyDeviceToHost, st));
+  HIPCHK(hipMemcpyAsync(lo.data(), (const char*)g.sol_lo.p + (size_t)member * n * 4, n * 4,
+                        hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  for (size_t i = 0; i < n; ++i) out[i] = (double)hi[i] + (double)lo[i];
+  return PTF_OK;
+}
+
+// set sol of one member from FP64 (split into hi + lo in the mixed form)
+static int set_member_sol(Group& g, int member, const double* in, hipStream_t st) {
+  const size_t n = (size_t)g.nS * 2;
+  if (g.dbl) {
+    HIPCHK(hipMemcpyAsync((char*)g.sol.p + (size_t)member * n * 8, in, n * 8,
+                          hipMemcpyHostToDevice, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return PTF_OK;
+  }
+  std::vector<float> hi(n), lo(n);
+  for (size_t i = 0; i < n; ++i) {
+    hi[i] = (float)in[i];
+    lo[i] = (float)(in[i] - (double)hi[i]);
+  }
+  HIPCHK(hipMemcpyAsync((char*)g.sol.p + (size_t)member * n * 4, hi.data(), n * 4,
+                        hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync((char*)g.sol_lo.p + (size_t)member * n * 4, lo.data(), n * 4,
+                        hipMemcpyHostToDevice, st));
+  HIPCHK(hipStreamSynchronize(st));
+  return PTF_OK;
+}
+
+// after a set_c the low part of the member's compensated state is zero
+static int clear_member_lo(Group& g, int member, hipStream_t st) {
+  if (!g.sol_lo.p) return PTF_OK;
+  const size_t nb = (size_t)g.nS * 2 * g.esz;
+  HIPCHK(hipMemsetAsync((char*)g.sol_lo.p + (size_t)member * nb, 0, nb, st));
+  return PTF_OK;
+}
 
 extern "C" {
 
@@ -1070,6 +1128,7 @@ int ptf_set_c(ptf_problem* prob, const double* c) {
   Group& g = prob->g;
   CHK(upload_real(g, g.phys.p, c, (size_t)g.nR, prob->stream));
   CHK(g_rfft(g, g.phys.p, g.sol.p, prob->stream));
+  CHK(clear_member_lo(g, 0, prob->stream));
   HIPCHK(hipStreamSynchronize(prob->stream));
   return PTF_OK;
 }
@@ -1109,14 +1168,12 @@ static int upload_spec(Group& g, void* dst, const double* src, hipStream_t st) {
 
 int ptf_get_sol(ptf_problem* prob, double* sol) {
   if (!prob || !sol) return fail(PTF_ERR_INVALID, "NULL argument");
-  return download_spec(prob->g, sol, prob->g.sol.p, prob->stream);
+  return get_member_sol(prob->g, 0, sol, prob->stream);
 }
 
 int ptf_set_sol(ptf_problem* prob, const double* sol) {
   if (!prob || !sol) return fail(PTF_ERR_INVALID, "NULL argument");
-  CHK(upload_spec(prob->g, prob->g.sol.p, sol, prob->stream));
-  HIPCHK(hipStreamSynchronize(prob->stream));
-  return PTF_OK;
+  return set_member_sol(prob->g, 0, sol, prob->stream);
 }
 
 int ptf_updatevars(ptf_problem* prob, double* c) {
@@ -1362,6 +1419,7 @@ int ptf_ensemble_set_c(ptf_ensemble* ens, int32_t member, const double* c) {
   CHK(upload_real(*g, g->phys.p, c, (size_t)g->nR, ens->stream));
   char* dst = (char*)g->sol.p + (size_t)i * g->nS * 2 * g->esz;
   CHK(g_rfft(*g, g->phys.p, dst, ens->stream));
+  CHK(clear_member_lo(*g, i, ens->stream));
   HIPCHK(hipStreamSynchronize(ens->stream));
   return PTF_OK;
 }
@@ -1381,8 +1439,7 @@ int ptf_ensemble_get_sol(ptf_ensemble* ens, int32_t member, double* sol) {
   int i;
   CHK(member_of(ens, member, &g, &i));
   if (!sol) return fail(PTF_ERR_INVALID, "sol is NULL");
-  char* src = (char*)g->sol.p + (size_t)i * g->nS * 2 * g->esz;
-  return download_spec(*g, sol, src, ens->stream);
+  return get_member_sol(*g, i, sol, ens->stream);
 }
 
 static int ensemble_record_step(ptf_ensemble* e) {

@@ -269,6 +269,7 @@ struct StageArgs {
   int kind;     // EPI_*
   int stage;    // 0..3
   cpx<T>* sol;  // state
+  cpx<T>* sol_lo;  // mixed form: low part of the compensated state (sol + sol_lo), else NULL
   cpx<T>* stg;  // RK4: stage input sol1;  ETDRK4: sol1
   cpx<T>* stg2; // ETDRK4: sol2
   cpx<T>* acc;  // RK4: running RHS sum; ETDRK4: partial update
@@ -286,6 +287,18 @@ struct StageArgs {
   GridInfo g;
   long long mstride;    // spectral elements per member
 };
+
+// (hi, lo) + y -> (hi', lo') in working precision: TwoSum of hi and y, the error
+// and lo folded together, then a FastTwoSum renormalisation (Knuth; Dekker).
+template <typename T>
+__device__ __forceinline__ void comp_add(T hi, T lo, T y, T& ho, T& lo_out) {
+  const T s = hi + y;
+  const T bb = s - hi;
+  const T err = (hi - (s - bb)) + (y - bb);
+  const T e = err + lo;
+  ho = s + e;
+  lo_out = e - (ho - s);
+}
 
 // Stage update at spectral element idx (member-relative wavenumber indices
 // ikx, iky, ikz) given the nonlinear term N of the stage input.
@@ -318,36 +331,71 @@ __device__ __forceinline__ void stage_epilogue(const StageArgs<T>& a, long long 
       cpx<T> s = a.sol[idx];
       cpx<T> ac = a.acc[idx];
       cpx<T> tot = {ac.re + rhs.re / (T)6, ac.im + rhs.im / (T)6};
-      a.sol[idx] = {s.re + dt * tot.re, s.im + dt * tot.im};
+      if (a.sol_lo) {
+        // compensated accumulation: (hi, lo) += dt tot with TwoSum + renormalisation
+        const cpx<T> lo = a.sol_lo[idx];
+        T hr, lr, hi_, li_;
+        comp_add(s.re, lo.re, dt * tot.re, hr, lr);
+        comp_add(s.im, lo.im, dt * tot.im, hi_, li_);
+        a.sol[idx] = {hr, hi_};
+        a.sol_lo[idx] = {lr, li_};
+      } else {
+        a.sol[idx] = {s.re + dt * tot.re, s.im + dt * tot.im};
+      }
     }
   } else if (a.kind == EPI_ETDRK4) {
     // coefficients are FP64 arrays; the update runs in FP64 and rounds once
     const double nre = (double)N.re, nim = (double)N.im;
-    if (a.stage == 0) {
+    auto sol_at = [&](double& re, double& im) {
       const cpx<T> s = a.sol[idx];
+      re = (double)s.re;
+      im = (double)s.im;
+      if (a.sol_lo) {
+        const cpx<T> lo = a.sol_lo[idx];
+        re += (double)lo.re;
+        im += (double)lo.im;
+      }
+    };
+    if (a.stage == 0 || a.stage == 1) {
+      double sr, si;
+      sol_at(sr, si);
       const double e2 = a.E2[idx], z = a.zeta[idx];
-      a.stg[idx] = {(T)(e2 * (double)s.re + z * nre), (T)(e2 * (double)s.im + z * nim)};
-      a.n1[idx] = N;
-    } else if (a.stage == 1) {
-      const cpx<T> s = a.sol[idx];
-      const double e2 = a.E2[idx], z = a.zeta[idx];
-      a.stg2[idx] = {(T)(e2 * (double)s.re + z * nre), (T)(e2 * (double)s.im + z * nim)};
-      a.n2[idx] = N;
+      const cpx<T> v = {(T)(e2 * sr + z * nre), (T)(e2 * si + z * nim)};
+      if (a.stage == 0) {
+        a.stg[idx] = v;
+        a.n1[idx] = N;
+      } else {
+        a.stg2[idx] = v;
+        a.n2[idx] = N;
+      }
     } else if (a.stage == 2) {
       const cpx<T> s1 = a.stg[idx];
-      const cpx<T> s = a.sol[idx];
+      double sr, si;
+      sol_at(sr, si);
       const cpx<T> N1 = a.n1[idx];
       const cpx<T> N2 = a.n2[idx];
       const double e2 = a.E2[idx], z = a.zeta[idx];
       const double e = a.E[idx], al = a.alpha[idx], be2 = 2.0 * a.beta[idx];
       a.stg2[idx] = {(T)(e2 * (double)s1.re + z * (2.0 * nre - (double)N1.re)),
                      (T)(e2 * (double)s1.im + z * (2.0 * nim - (double)N1.im))};
-      a.acc[idx] = {(T)(e * (double)s.re + al * (double)N1.re + be2 * ((double)N2.re + nre)),
-                    (T)(e * (double)s.im + al * (double)N1.im + be2 * ((double)N2.im + nim))};
+      const double ar = e * sr + al * (double)N1.re + be2 * ((double)N2.re + nre);
+      const double ai = e * si + al * (double)N1.im + be2 * ((double)N2.im + nim);
+      const cpx<T> hi = {(T)ar, (T)ai};
+      a.acc[idx] = hi;
+      if (a.sol_lo) a.n2[idx] = {(T)(ar - (double)hi.re), (T)(ai - (double)hi.im)};  // low part
     } else {
       const cpx<T> ac = a.acc[idx];
+      double ar = (double)ac.re, ai = (double)ac.im;
+      if (a.sol_lo) {
+        const cpx<T> al = a.n2[idx];
+        ar += (double)al.re;
+        ai += (double)al.im;
+      }
       const double g = a.gam[idx];
-      a.sol[idx] = {(T)((double)ac.re + g * nre), (T)((double)ac.im + g * nim)};
+      const double sr = ar + g * nre, si = ai + g * nim;
+      const cpx<T> hi = {(T)sr, (T)si};
+      a.sol[idx] = hi;
+      if (a.sol_lo) a.sol_lo[idx] = {(T)(sr - (double)hi.re), (T)(si - (double)hi.im)};
     }
   }
 }

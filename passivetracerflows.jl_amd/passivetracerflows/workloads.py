@@ -171,3 +171,31 @@ def ensemble_2d(total=512, seed=5, dt=5e-3, nsteps=1000, counts=None):
             idx += 1
     return dict(name=f"2D cellular-flow ensemble of {len(members)}", members=members, dt=dt,
                 nsteps=nsteps, stepper="RK4", counts=counts)
+
+
+RK4_REAL_LIMIT = 2.785293563405282  # RK4 stability interval on the negative real axis
+
+
+def spectral_tail(solh, n):
+    """Fraction of the L2 norm of a 2D half spectrum held by modes with |kx| or |ky| > n/3
+    (the band the 2/3 rule would remove); a resolved field keeps this small."""
+    kx = np.arange(n // 2 + 1)[None, :]
+    ky = np.abs(np.fft.fftfreq(n, 1.0 / n))[:, None]
+    mask = np.broadcast_to((kx > n / 3) | (ky > n / 3), solh.shape)
+    tot = np.linalg.norm(solh)
+    return float(np.linalg.norm(solh[mask]) / tot) if tot > 0 else 0.0
+
+
+def member_class(m, dt, sol64, tail_limit=1e-2):
+    """'diffusion_unstable': kappa |k_max|^2 dt beyond RK4's real-axis limit (the FP64 run
+    overflows, as the reference algorithm does); 'under_resolved': the FP64 state at the end
+    holds more than 1 % of its norm in the |k| > n/3 band (grid-scale noise of the
+    non-dealiased scheme); otherwise 'resolved'."""
+    n = m["n"]
+    if m["kappa"] * 2 * (n / 2) ** 2 * dt > RK4_REAL_LIMIT:
+        return "diffusion_unstable"
+    with np.errstate(over="ignore", invalid="ignore"):
+        tail = spectral_tail(sol64, n) if np.all(np.isfinite(sol64)) else float("nan")
+    if not tail <= tail_limit:   # NaN (overflowed norm) counts as under-resolved
+        return "under_resolved"
+    return "resolved"

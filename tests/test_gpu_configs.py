@@ -1,0 +1,168 @@
+"""The measured workloads at their real sizes.
+
+Where the oracle finishes in seconds the device is compared with it after the
+configured steps (FP64 relative L2 <= 1e-12; mixed <= 1e-5).  Where it does
+not (3D 256^3 x 500, 2D 2048^2 x 1000, 512 members x 1000 steps), the device
+is compared with the oracle on a shortened run and the full run is checked by
+size-independent properties: mass conservation to roundoff, non-increasing
+variance, mixed-vs-FP64 agreement, and the closed-form discrete solution where
+one exists.
+"""
+import numpy as np
+import pytest
+
+import passivetracerflows as ptf
+from oracle import tad as O
+from passivetracerflows import workloads as W
+
+pytestmark = pytest.mark.gpu
+TAD = ptf.TracerAdvectionDiffusion
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.ravel(a - b)) / np.linalg.norm(np.ravel(b)))
+
+
+def _fl(fields):
+    return [lambda *a, f=f: f for f in fields]
+
+
+def test_gaussian_1d_full(dev):
+    """1D Gaussian diffusion, 128 points, RK4 x 1000: oracle and the closed form
+    c_n = R(dt L)^n c_0 (u = 0 makes the scheme exactly linear and diagonal)."""
+    cfg = W.gaussian_1d()
+    p = TAD.Problem(dev, ptf.OneDAdvectingFlow(), nx=128, κ=cfg["kappa"], dt=cfg["dt"])
+    TAD.set_c_(p, cfg["c0"])
+    ptf.stepforward_(p, cfg["nsteps"])
+    g = O.Grid(1, 128)
+    o = O.Problem(g, cfg["flow"], kappa=cfg["kappa"], dt=cfg["dt"])
+    o.set_c(cfg["c0"])
+    o.stepforward(cfg["nsteps"])
+    assert rel(p.sol, o.sol) <= 1e-12
+    z = cfg["dt"] * o.L
+    closed = (1 + z + z ** 2 / 2 + z ** 3 / 6 + z ** 4 / 24) ** cfg["nsteps"] * O.rfft(g, cfg["c0"])
+    assert rel(p.sol, closed) <= 1e-12
+
+
+@pytest.mark.parametrize("T,tol", [(np.float64, 1e-12), (np.float32, 1e-5)])
+def test_cellular_2d_full(dev, T, tol):
+    """2D cellular flow 256^2, RK4 x 2000 steps, against the FP64 oracle."""
+    cfg = W.cellular_2d(256)
+    print(f"cellular 256^2 x {cfg['nsteps']} {T.__name__}", flush=True)
+    u, v = cfg["flow"]
+    p = TAD.Problem(dev, ptf.TwoDAdvectingFlow(*_fl((u, v))), nx=256, κ=cfg["kappa"],
+                    dt=cfg["dt"], T=T)
+    TAD.set_c_(p, cfg["c0"])
+    m0 = TAD.mass(p)
+    ptf.stepforward_(p, cfg["nsteps"])
+    o = O.Problem(O.Grid(2, 256), (u, v), kappa=cfg["kappa"], dt=cfg["dt"])
+    o.set_c(cfg["c0"])
+    o.stepforward(cfg["nsteps"])
+    assert rel(p.sol, o.sol) <= tol
+    assert abs(TAD.mass(p) - m0) <= 1e-9 * np.abs(cfg["c0"]).sum()
+
+
+def test_abc_3d_short_parity_and_full_properties(dev):
+    cfg = W.abc_3d(256)
+    u, v, w = cfg["flow"]
+    mk = lambda T: TAD.Problem(dev, ptf.ThreeDAdvectingFlow(*_fl((u, v, w))), nx=256,
+                               κ=cfg["kappa"], dt=cfg["dt"], T=T)
+    p = mk(np.float64)
+    TAD.set_c_(p, cfg["c0"])
+    ptf.stepforward_(p, 3)
+    print("abc 256^3: oracle 3 steps", flush=True)
+    o = O.Problem(O.Grid(3, 256), (u, v, w), kappa=cfg["kappa"], dt=cfg["dt"])
+    o.set_c(cfg["c0"])
+    o.stepforward(3)
+    assert rel(p.sol, o.sol) <= 1e-12
+    # full run: properties
+    l1 = np.abs(cfg["c0"]).sum()
+    m0, v0 = TAD.mass(p), TAD.variance(p)
+    q = mk(np.float32)
+    TAD.set_c_(q, cfg["c0"])
+    ptf.stepforward_(q, 3)
+    vprev = v0
+    for _ in range(cfg["nsteps"] // 50 - 1 if cfg["nsteps"] >= 100 else 1):
+        ptf.stepforward_(p, 50)
+        ptf.stepforward_(q, 50)
+        vv = TAD.variance(p)
+        assert vv <= vprev * (1 + 1e-9)   # d/dt int c^2 = -2 kappa int |grad c|^2 <= 0
+        vprev = vv
+    rest = cfg["nsteps"] - p.clock.step
+    ptf.stepforward_(p, rest)
+    ptf.stepforward_(q, rest)
+    assert p.clock.step == cfg["nsteps"]
+    assert abs(TAD.mass(p) - m0) <= 1e-12 * l1
+    assert rel(q.sol, p.sol) <= 1e-5
+
+
+def test_turbulent_2d_short_parity(dev):
+    """2048^2 random-phase flow, ETDRK4, hyperdiffusion order 4: 10 steps vs the oracle."""
+    cfg = W.turbulent_2d(2048)
+    n, nsteps = 2048, 10
+    flow = lambda t: W.spectral_flow_at(cfg["psih0"], cfg["omega"], t, n)
+    o = O.Problem(O.Grid(2, n), flow, kappa=0.0, eta=0.0, kappah=cfg["kappah"],
+                  nkappah=cfg["nkappah"], dt=cfg["dt"], stepper="ETDRK4")
+    o.set_c(cfg["c0"])
+    o.stepforward(nsteps)
+    for T, tol in ((np.float64, 1e-12), (np.float32, 1e-5)):
+        p = TAD.Problem(dev, ptf.SpectralStreamfunctionFlow(cfg["psih0"], cfg["omega"]), nx=n,
+                        κ=0.0, η=0.0, κh=cfg["kappah"], nκh=cfg["nkappah"], dt=cfg["dt"],
+                        stepper="ETDRK4", T=T)
+        TAD.set_c_(p, cfg["c0"])
+        ptf.stepforward_(p, nsteps)
+        assert rel(p.sol, o.sol) <= tol, T
+
+
+def test_turbulent_2d_full_properties(dev):
+    cfg = W.turbulent_2d(2048)
+    ps = []
+    for T in (np.float64, np.float32):
+        p = TAD.Problem(dev, ptf.SpectralStreamfunctionFlow(cfg["psih0"], cfg["omega"]), nx=2048,
+                        κ=0.0, η=0.0, κh=cfg["kappah"], nκh=cfg["nkappah"], dt=cfg["dt"],
+                        stepper="ETDRK4", T=T)
+        TAD.set_c_(p, cfg["c0"])
+        ps.append(p)
+    m0 = TAD.mass(ps[0])
+    for p in ps:
+        ptf.stepforward_(p, cfg["nsteps"])
+    assert abs(TAD.mass(ps[0]) - m0) <= 1e-10 * np.abs(cfg["c0"]).sum()
+    assert rel(ps[1].sol, ps[0].sol) <= 1e-5
+
+
+def test_ensemble_512_full(dev):
+    """512 members x 1000 steps: each FP32 member against its own FP64 run; a sample of
+    members against the oracle."""
+    cfg = W.ensemble_2d()
+    members = [dict(m, **W.ensemble_member(m["n"], m["seed"])) for m in cfg["members"]]
+    ens = {}
+    for prec in ("mixed", "fp64"):
+        e = ptf.Ensemble([dict(n=m["n"], kappa=m["kappa"]) for m in members], dt=cfg["dt"],
+                         precision=prec)
+        for i, m in enumerate(members):
+            e.set_flow(i, *m["flow"])
+            e.set_c(i, m["c0"])
+        e.step(cfg["nsteps"], graph=True)
+        ens[prec] = e
+    counts = {"resolved": 0, "under_resolved": 0, "diffusion_unstable": 0}
+    for i, m in enumerate(members):
+        b = ens["fp64"].get_sol(i)
+        cls = W.member_class(m, cfg["dt"], b)
+        counts[cls] += 1
+        if cls == "resolved":
+            assert rel(ens["mixed"].get_sol(i), b) <= 1e-5, i
+    print(counts, flush=True)
+    assert counts["diffusion_unstable"] == 6
+    assert counts["resolved"] >= 100
+    # members of each size against the oracle (the first member of each size class)
+    firsts = {}
+    for i, m in enumerate(members):
+        firsts.setdefault(m["n"], i)
+    for n, i in firsts.items():
+        if n > 256:
+            continue
+        m = members[i]
+        o = O.Problem(O.Grid(2, n), m["flow"], kappa=m["kappa"], dt=cfg["dt"])
+        o.set_c(m["c0"])
+        o.stepforward(cfg["nsteps"])
+        assert rel(ens["fp64"].get_sol(i), o.sol) <= 1e-12, i
