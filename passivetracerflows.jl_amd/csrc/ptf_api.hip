@@ -13,6 +13,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -87,8 +88,10 @@ static const char* kind_name(int kind) {
 }
 
 static const int kRadices[] = {16, 8, 4};
-static const int kColWidths[] = {16, 8, 32, 4, 2};
-static const int kRowWidths[] = {8, 16, 4, 2, 1};
+static const int kColWidths[] = {16, 8, 32, 64, 4, 2};
+static const int kNumColWidths = 6;
+static const int kRowWidths[] = {8, 16, 4, 2, 1, 1};
+static const int kNumRowWidths = 5;
 
 // ---------------------------------------------------------------------------
 // launch slots
@@ -128,8 +131,9 @@ static bool choose_variant(Slot& s, int R, int W) {
 // default variant: first valid (R, W) in preference order
 static int pick_default(Slot& s) {
   const int* widths = s.is_col() ? kColWidths : kRowWidths;
+  const int nw = s.is_col() ? kNumColWidths : kNumRowWidths;
   for (int R : kRadices) {
-    for (int wi = 0; wi < 5; ++wi) {
+    for (int wi = 0; wi < nw; ++wi) {
       int W = widths[wi];
       if (!s.is_col() && s.lines < W && W > 1) continue;  // no idle rows
       if (choose_variant(s, R, W)) return PTF_OK;
@@ -137,7 +141,7 @@ static int pick_default(Slot& s) {
   }
   // fall back to any width
   for (int R : kRadices)
-    for (int wi = 0; wi < 5; ++wi)
+    for (int wi = 0; wi < nw; ++wi)
       if (choose_variant(s, R, widths[wi])) return PTF_OK;
   return fail(PTF_ERR_UNSUPPORTED, std::string("no kernel variant for ") + kind_name(s.kind) +
                                         " with line length " + std::to_string(s.N));
@@ -844,14 +848,19 @@ static int autotune_group(Group& g, hipStream_t st) {
     if (std::find(done.begin(), done.end(), key) != done.end()) continue;
     done.push_back(key);
     const int* widths = s->is_col() ? kColWidths : kRowWidths;
+    const int nw = s->is_col() ? kNumColWidths : kNumRowWidths;
     int bestR = s->R, bestW = s->W;
     double best = 1e30;
+    const bool verbose = std::getenv("PTF_AUTOTUNE_VERBOSE") != nullptr;
     for (int R : kRadices) {
-      for (int wi = 0; wi < 5; ++wi) {
+      for (int wi = 0; wi < nw; ++wi) {
         Slot trial = *s;
         if (!choose_variant(trial, R, widths[wi])) continue;
         double ms = 0;
         CHK(time_slot(g, trial, 5, st, &ms));
+        if (verbose)
+          std::fprintf(stderr, "autotune %s N=%d R=%d W=%d: %.4f ms\n", kind_name(trial.kind),
+                       trial.N, R, widths[wi], ms);
         if (ms < best) {
           best = ms;
           bestR = R;

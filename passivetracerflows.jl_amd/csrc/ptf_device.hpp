@@ -300,8 +300,9 @@ __device__ __forceinline__ void comp_add(T hi, T lo, T y, T& ho, T& lo_out) {
   lo_out = e - (ho - s);
 }
 
-// Stage update at spectral element idx (member-relative wavenumber indices
-// ikx, iky, ikz) given the nonlinear term N of the stage input.
+// Stage update at spectral element ubase + off (ubase uniform over the block,
+// off a 32-bit lane offset; member-relative wavenumber indices ikx, iky, ikz)
+// given the nonlinear term N of the stage input.
 // RK4 ([FF] RK4substeps!/RK4update!): RHS_s = N_s + L x_s;
 //   x_2 = sol + dt/2 RHS_1, x_3 = sol + dt/2 RHS_2, x_4 = sol + dt RHS_3,
 //   sol += dt (RHS_1/6 + RHS_2/3 + RHS_3/3 + RHS_4/6)   (sum in this order)
@@ -309,49 +310,49 @@ __device__ __forceinline__ void comp_add(T hi, T lo, T y, T& ho, T& lo_out) {
 //   sol1 = E2 sol + zeta N1 ; sol2 = E2 sol + zeta N2 ; sol2 = E2 sol1 + zeta (2 N3 - N1);
 //   sol = E sol + alpha N1 + 2 beta (N2 + N3) + Gamma N4
 template <typename T>
-__device__ __forceinline__ void stage_epilogue(const StageArgs<T>& a, long long idx, int member,
+__device__ __forceinline__ void stage_epilogue(const StageArgs<T>& a, long long ubase, uint32_t off, int member,
                                                int ikx, int iky, int ikz, cpx<T> N) {
   if (a.kind == EPI_RK4) {
     const double Ld = linear_op(a.g, a.op[member], a.nkappah, ikx, iky, ikz);
     const T L = (T)Ld;
     const T dt = (T)a.dt;
-    cpx<T> x = a.stage == 0 ? a.sol[idx] : a.stg[idx];
+    cpx<T> x = a.stage == 0 ? (a.sol + ubase)[off] : (a.stg + ubase)[off];
     cpx<T> rhs = {N.re + L * x.re, N.im + L * x.im};
     if (a.stage == 0) {
       const T h = (T)(a.dt / 2);
-      a.stg[idx] = {x.re + h * rhs.re, x.im + h * rhs.im};
-      a.acc[idx] = {rhs.re / (T)6, rhs.im / (T)6};
+      (a.stg + ubase)[off] = {x.re + h * rhs.re, x.im + h * rhs.im};
+      (a.acc + ubase)[off] = {rhs.re / (T)6, rhs.im / (T)6};
     } else if (a.stage == 1 || a.stage == 2) {
       const T h = a.stage == 1 ? (T)(a.dt / 2) : dt;
-      cpx<T> s = a.sol[idx];
-      cpx<T> ac = a.acc[idx];
-      a.stg[idx] = {s.re + h * rhs.re, s.im + h * rhs.im};
-      a.acc[idx] = {ac.re + rhs.re / (T)3, ac.im + rhs.im / (T)3};
+      cpx<T> s = (a.sol + ubase)[off];
+      cpx<T> ac = (a.acc + ubase)[off];
+      (a.stg + ubase)[off] = {s.re + h * rhs.re, s.im + h * rhs.im};
+      (a.acc + ubase)[off] = {ac.re + rhs.re / (T)3, ac.im + rhs.im / (T)3};
     } else {
-      cpx<T> s = a.sol[idx];
-      cpx<T> ac = a.acc[idx];
+      cpx<T> s = (a.sol + ubase)[off];
+      cpx<T> ac = (a.acc + ubase)[off];
       cpx<T> tot = {ac.re + rhs.re / (T)6, ac.im + rhs.im / (T)6};
       if (a.sol_lo) {
         // compensated accumulation: (hi, lo) += dt tot with TwoSum + renormalisation
-        const cpx<T> lo = a.sol_lo[idx];
+        const cpx<T> lo = (a.sol_lo + ubase)[off];
         T hr, lr, hi_, li_;
         comp_add(s.re, lo.re, dt * tot.re, hr, lr);
         comp_add(s.im, lo.im, dt * tot.im, hi_, li_);
-        a.sol[idx] = {hr, hi_};
-        a.sol_lo[idx] = {lr, li_};
+        (a.sol + ubase)[off] = {hr, hi_};
+        (a.sol_lo + ubase)[off] = {lr, li_};
       } else {
-        a.sol[idx] = {s.re + dt * tot.re, s.im + dt * tot.im};
+        (a.sol + ubase)[off] = {s.re + dt * tot.re, s.im + dt * tot.im};
       }
     }
   } else if (a.kind == EPI_ETDRK4) {
     // coefficients are FP64 arrays; the update runs in FP64 and rounds once
     const double nre = (double)N.re, nim = (double)N.im;
     auto sol_at = [&](double& re, double& im) {
-      const cpx<T> s = a.sol[idx];
+      const cpx<T> s = (a.sol + ubase)[off];
       re = (double)s.re;
       im = (double)s.im;
       if (a.sol_lo) {
-        const cpx<T> lo = a.sol_lo[idx];
+        const cpx<T> lo = (a.sol_lo + ubase)[off];
         re += (double)lo.re;
         im += (double)lo.im;
       }
@@ -359,43 +360,43 @@ __device__ __forceinline__ void stage_epilogue(const StageArgs<T>& a, long long 
     if (a.stage == 0 || a.stage == 1) {
       double sr, si;
       sol_at(sr, si);
-      const double e2 = a.E2[idx], z = a.zeta[idx];
+      const double e2 = (a.E2 + ubase)[off], z = (a.zeta + ubase)[off];
       const cpx<T> v = {(T)(e2 * sr + z * nre), (T)(e2 * si + z * nim)};
       if (a.stage == 0) {
-        a.stg[idx] = v;
-        a.n1[idx] = N;
+        (a.stg + ubase)[off] = v;
+        (a.n1 + ubase)[off] = N;
       } else {
-        a.stg2[idx] = v;
-        a.n2[idx] = N;
+        (a.stg2 + ubase)[off] = v;
+        (a.n2 + ubase)[off] = N;
       }
     } else if (a.stage == 2) {
-      const cpx<T> s1 = a.stg[idx];
+      const cpx<T> s1 = (a.stg + ubase)[off];
       double sr, si;
       sol_at(sr, si);
-      const cpx<T> N1 = a.n1[idx];
-      const cpx<T> N2 = a.n2[idx];
-      const double e2 = a.E2[idx], z = a.zeta[idx];
-      const double e = a.E[idx], al = a.alpha[idx], be2 = 2.0 * a.beta[idx];
-      a.stg2[idx] = {(T)(e2 * (double)s1.re + z * (2.0 * nre - (double)N1.re)),
+      const cpx<T> N1 = (a.n1 + ubase)[off];
+      const cpx<T> N2 = (a.n2 + ubase)[off];
+      const double e2 = (a.E2 + ubase)[off], z = (a.zeta + ubase)[off];
+      const double e = (a.E + ubase)[off], al = (a.alpha + ubase)[off], be2 = 2.0 * (a.beta + ubase)[off];
+      (a.stg2 + ubase)[off] = {(T)(e2 * (double)s1.re + z * (2.0 * nre - (double)N1.re)),
                      (T)(e2 * (double)s1.im + z * (2.0 * nim - (double)N1.im))};
       const double ar = e * sr + al * (double)N1.re + be2 * ((double)N2.re + nre);
       const double ai = e * si + al * (double)N1.im + be2 * ((double)N2.im + nim);
       const cpx<T> hi = {(T)ar, (T)ai};
-      a.acc[idx] = hi;
-      if (a.sol_lo) a.n2[idx] = {(T)(ar - (double)hi.re), (T)(ai - (double)hi.im)};  // low part
+      (a.acc + ubase)[off] = hi;
+      if (a.sol_lo) (a.n2 + ubase)[off] = {(T)(ar - (double)hi.re), (T)(ai - (double)hi.im)};  // low part
     } else {
-      const cpx<T> ac = a.acc[idx];
+      const cpx<T> ac = (a.acc + ubase)[off];
       double ar = (double)ac.re, ai = (double)ac.im;
       if (a.sol_lo) {
-        const cpx<T> al = a.n2[idx];
+        const cpx<T> al = (a.n2 + ubase)[off];
         ar += (double)al.re;
         ai += (double)al.im;
       }
-      const double g = a.gam[idx];
+      const double g = (a.gam + ubase)[off];
       const double sr = ar + g * nre, si = ai + g * nim;
       const cpx<T> hi = {(T)sr, (T)si};
-      a.sol[idx] = hi;
-      if (a.sol_lo) a.sol_lo[idx] = {(T)(sr - (double)hi.re), (T)(si - (double)hi.im)};
+      (a.sol + ubase)[off] = hi;
+      if (a.sol_lo) (a.sol_lo + ubase)[off] = {(T)(sr - (double)hi.re), (T)(si - (double)hi.im)};
     }
   }
 }

@@ -45,104 +45,157 @@ struct ColArgs {
   double kscale_x;
 };
 
+// Occupancy request for a block of `threads` threads using `lds` bytes of LDS:
+// the waves per SIMD that the LDS budget (160 KiB per CU) admits, capped by
+// what the line registers of radix R in precision T need (so that the compiler
+// does not spill), in [1, 8].
+template <typename T>
+constexpr int occupancy_waves(int threads, int lds, int R, int live_arrays) {
+  int blocks = lds > 0 ? 163840 / lds : 16;
+  if (blocks < 1) blocks = 1;
+  if (blocks > 16) blocks = 16;
+  int w = blocks * ((threads + 63) / 64) / 4;
+  const int regs = live_arrays * R * (int)(2 * sizeof(T) / 4) + 48;
+  const int wr = 512 / regs;
+  if (w > wr) w = wr;
+  return w < 1 ? 1 : (w > 8 ? 8 : w);
+}
+
+// Workgroups are dispatched round-robin over the 8 XCDs of the device, each
+// with its own L2.  Renumber the grid so that XCD k runs a contiguous range of
+// logical blocks (x fastest, then y, then z): neighbouring tiles, which share
+// rows of the same planes, then meet in one L2.
+__device__ __forceinline__ void xcd_remap(int& bx, int& by, int& bz) {
+  constexpr unsigned kXcd = 8;
+  const unsigned gx = gridDim.x, gy = gridDim.y;
+  const unsigned n = gx * gy * gridDim.z;
+  const unsigned L = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
+  const unsigned xcd = L % kXcd, local = L / kXcd;
+  const unsigned per = n / kXcd, rem = n % kXcd;
+  const unsigned logical = xcd * per + (xcd < rem ? xcd : rem) + local;
+  bx = (int)(logical % gx);
+  by = (int)((logical / gx) % gy);
+  bz = (int)(logical / (gx * gy));
+}
+
+template <typename T, int N, int R, int TX>
+constexpr int col_lds_elems() { return (N > R) ? N * TX : 1; }
+
 template <typename T, int N, int R, int TX, int MODE, int DIR>
-__global__ __launch_bounds__(TX*(N / R)) void col_kernel(ColArgs<T> a) {
+__global__ __attribute__((amdgpu_flat_work_group_size(1, TX * (N / R)),
+                          amdgpu_waves_per_eu(occupancy_waves<T>(TX * (N / R),
+                                              (int)(col_lds_elems<T, N, R, TX>() * sizeof(cpx<T>)),
+                                              R, 3))))
+void col_kernel(ColArgs<T> a) {
   constexpr int P = N / R;
   using Lay = LayInterleaved<TX>;
-  __shared__ cpx<T> lds[(N > R) ? N * TX : 1];
+  __shared__ cpx<T> lds[col_lds_elems<T, N, R, TX>()];
   const int t = threadIdx.x;
   const int c = t % TX;
   const int j = t / TX;
-  const int kx = blockIdx.x * TX + c;
+  int bx, by, member;
+  xcd_remap(bx, by, member);
+  const int kx = bx * TX + c;
   const bool valid = kx < a.ncols;
-  const long long o = blockIdx.y;
-  const int member = blockIdx.z;
-  const long long base = (long long)member * a.mstride + o * a.ostride + kx;
-  const long long s = a.stride;
-  const cpx<T> zero = {(T)0, (T)0};
+  const int kxc = valid ? kx : a.ncols - 1;   // invalid lanes load a valid column, store nothing
+  // block-uniform base (SGPRs) + 32-bit lane offsets
+  const long long ubase = (long long)member * a.mstride + (long long)by * a.ostride;
+  const uint32_t s = (uint32_t)a.stride;
+  auto off = [&](int q) -> uint32_t { return (uint32_t)kxc + (uint32_t)(j + q * P) * s; };
 
   cpx<T> v[R];
   if constexpr (MODE == COL_FLOWSYN) {
-    cpx<T> w[R];
+    const cpx<double>* ps = a.psih0 + ubase;
+    const double* om = a.omega + ubase;
+    // out1 = IFFT_y(i kr psi(t)) first, then out0 = IFFT_y(-i l psi(t))
 #pragma unroll
-    for (int q = 0; q < R; ++q) {
-      const int iy = j + q * P;
-      if (valid) {
-        const cpx<double> p0 = a.psih0[base + iy * s];
-        const double ph = -a.omega[base + iy * s] * a.t;
+    for (int pass = 0; pass < 2; ++pass) {
+#pragma unroll
+      for (int q = 0; q < R; ++q) {
+        const int iy = j + q * P;
+        const cpx<double> p0 = ps[off(q)];
+        const double ph = -om[off(q)] * a.t;
         double sn, cs;
         sincos(ph, &sn, &cs);
         const cpx<double> p = {p0.re * cs - p0.im * sn, p0.re * sn + p0.im * cs};
-        const double l = fftfreq_index(iy, N) * a.kscale;
-        const double kr = kx * a.kscale_x;
-        // -i l p = (l p.im, -l p.re);  i kr p = (-kr p.im, kr p.re)
-        v[q] = {(T)(l * p.im), (T)(-(l * p.re))};
-        w[q] = {(T)(-(kr * p.im)), (T)(kr * p.re)};
-      } else {
-        v[q] = zero;
-        w[q] = zero;
+        if (pass == 0) {
+          const double kr = kx * a.kscale_x;
+          v[q] = {(T)(-(kr * p.im)), (T)(kr * p.re)};          // i kr p
+        } else {
+          const double l = fftfreq_index(iy, N) * a.kscale;
+          v[q] = {(T)(l * p.im), (T)(-(l * p.re))};             // -i l p
+        }
       }
-    }
-    line_fft<T, N, R, +1, Lay>(v, lds, c, j, a.tw, a.twstride);
-    if (valid) {
+      line_fft<T, N, R, +1, Lay>(v, lds, c, j, a.tw, a.twstride);
+      cpx<T>* o = (pass == 0 ? a.out1 : a.out0) + ubase;
+      if (valid) {
 #pragma unroll
-      for (int q = 0; q < R; ++q) a.out0[base + (j + q * P) * s] = v[q];
-    }
-    line_fft<T, N, R, +1, Lay>(w, lds, c, j, a.tw, a.twstride);
-    if (valid) {
-#pragma unroll
-      for (int q = 0; q < R; ++q) a.out1[base + (j + q * P) * s] = w[q];
+        for (int q = 0; q < R; ++q) o[off(q)] = v[q];
+      }
     }
     return;
   } else {
+    const cpx<T>* in0 = a.in0 + ubase;
 #pragma unroll
-    for (int q = 0; q < R; ++q) v[q] = valid ? a.in0[base + (j + q * P) * s] : zero;
+    for (int q = 0; q < R; ++q) v[q] = in0[off(q)];
 
     if constexpr (MODE == COL_PLAIN) {
       line_fft<T, N, R, DIR, Lay>(v, lds, c, j, a.tw, a.twstride);
+      cpx<T>* o = a.out0 + ubase;
       if (valid) {
 #pragma unroll
-        for (int q = 0; q < R; ++q) a.out0[base + (j + q * P) * s] = v[q];
+        for (int q = 0; q < R; ++q) o[off(q)] = v[q];
       }
     } else if constexpr (MODE == COL_D1 || MODE == COL_D2) {
-      cpx<T> w[R];
+      // derivative first: out0 may alias in0 (in place), so in0 is re-read
+      // for the plain transform only after out1 has been written
 #pragma unroll
       for (int q = 0; q < R; ++q) {
         const T k = (T)(fftfreq_index(j + q * P, N) * a.kscale);
-        w[q] = times_ik(v[q], k);
+        v[q] = times_ik(v[q], k);
       }
       line_fft<T, N, R, +1, Lay>(v, lds, c, j, a.tw, a.twstride);
-      if (valid) {
-#pragma unroll
-        for (int q = 0; q < R; ++q) a.out0[base + (j + q * P) * s] = v[q];
-      }
-      line_fft<T, N, R, +1, Lay>(w, lds, c, j, a.tw, a.twstride);
-      if (valid) {
-#pragma unroll
-        for (int q = 0; q < R; ++q) a.out1[base + (j + q * P) * s] = w[q];
-      }
-      if constexpr (MODE == COL_D2) {
-#pragma unroll
-        for (int q = 0; q < R; ++q) v[q] = valid ? a.in1[base + (j + q * P) * s] : zero;
-        line_fft<T, N, R, +1, Lay>(v, lds, c, j, a.tw, a.twstride);
+      {
+        cpx<T>* o = a.out1 + ubase;
         if (valid) {
 #pragma unroll
-          for (int q = 0; q < R; ++q) a.out2[base + (j + q * P) * s] = v[q];
+          for (int q = 0; q < R; ++q) o[off(q)] = v[q];
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < R; ++q) v[q] = in0[off(q)];
+      line_fft<T, N, R, +1, Lay>(v, lds, c, j, a.tw, a.twstride);
+      {
+        cpx<T>* o = a.out0 + ubase;
+        if (valid) {
+#pragma unroll
+          for (int q = 0; q < R; ++q) o[off(q)] = v[q];
+        }
+      }
+      if constexpr (MODE == COL_D2) {
+        const cpx<T>* in1 = a.in1 + ubase;
+#pragma unroll
+        for (int q = 0; q < R; ++q) v[q] = in1[off(q)];
+        line_fft<T, N, R, +1, Lay>(v, lds, c, j, a.tw, a.twstride);
+        cpx<T>* o = a.out2 + ubase;
+        if (valid) {
+#pragma unroll
+          for (int q = 0; q < R; ++q) o[off(q)] = v[q];
         }
       }
     } else if constexpr (MODE == COL_STAGE) {
       line_fft<T, N, R, -1, Lay>(v, lds, c, j, a.tw, a.twstride);
       if (valid) {
+        const int o = by;
 #pragma unroll
         for (int q = 0; q < R; ++q) {
           const int ia = j + q * P;
-          const long long idx = base + ia * s;
           if (a.st.kind == EPI_STORE) {
-            a.out0[idx] = v[q];
+            (a.out0 + ubase)[off(q)] = v[q];
           } else if (a.axis == 2) {
-            stage_epilogue(a.st, idx, member, kx, (int)o, ia, v[q]);
+            stage_epilogue(a.st, ubase, off(q), member, kx, o, ia, v[q]);
           } else {
-            stage_epilogue(a.st, idx, member, kx, ia, (int)o, v[q]);
+            stage_epilogue(a.st, ubase, off(q), member, kx, ia, o, v[q]);
           }
         }
       }
@@ -206,12 +259,35 @@ __device__ __forceinline__ cpx<T> r2c_post(cpx<T> Zk, cpx<T> Zm, cpx<T> w) {
   return {(T)0.5 * (e.re + od.im), (T)0.5 * (e.im - od.re)};
 }
 
+// Load spectral row entries X[k], k = j + q P, into x[] and the partners
+// X[M - k] into xm[] through LDS (the row's last entry X[M] is written by
+// lane j = 0).  Starts and ends with a barrier: LDS is free afterwards.
+template <typename T, int M, int R, class Lay>
+__device__ __forceinline__ void load_row_pairs(const cpx<T>* __restrict__ row, cpx<T> (&x)[R],
+                                               cpx<T> (&xm)[R], cpx<T>* lds, int c, int j) {
+  constexpr int P = M / R;
+#pragma unroll
+  for (int q = 0; q < R; ++q) x[q] = row[j + q * P];
+  cpx<T> last;
+  if (j == 0) last = row[M];
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < R; ++q) lds[Lay::idx(c, j + q * P)] = x[q];
+  if (j == 0) lds[Lay::idx(c, M)] = last;
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < R; ++q) xm[q] = lds[Lay::idx(c, M - (j + q * P))];
+}
+
 template <typename T, int NX, int R, int NL, int MODE, int NIN>
-__global__ __launch_bounds__(NL*((NX / 2) / R)) void row_kernel(RowArgs<T> a) {
+__global__ __attribute__((amdgpu_flat_work_group_size(1, NL * ((NX / 2) / R)),
+                          amdgpu_waves_per_eu(occupancy_waves<T>(NL * ((NX / 2) / R),
+                                              (int)(NL * (NX / 2 + 1) * sizeof(cpx<T>)), R, 5))))
+void row_kernel(RowArgs<T> a) {
   constexpr int M = NX / 2;
   constexpr int NXH = M + 1;
   constexpr int P = M / R;
-  constexpr int PAD = 1;
+  constexpr int PAD = 1;   // row stride M + 1 also holds X[M] while staging a row
   using Lay = LayRows<M, PAD>;
   __shared__ cpx<T> lds[NL * (M + PAD)];
   const int t = threadIdx.x;
@@ -219,69 +295,62 @@ __global__ __launch_bounds__(NL*((NX / 2) / R)) void row_kernel(RowArgs<T> a) {
   const int c = t / P;
   const long long row = (long long)blockIdx.x * NL + c;
   const bool valid = row < a.nrows;
+  const int cc = valid ? c : (int)(a.nrows - 1 - (long long)blockIdx.x * NL);  // clamped row
   const int member = blockIdx.y;
-  const long long sbase = (long long)member * a.smstride + row * NXH;
-  const long long pbase = (long long)member * a.pmstride + row * NX;
+  // block-uniform bases + 32-bit lane offsets
+  const long long sub = (long long)member * a.smstride + (long long)blockIdx.x * NL * NXH;
+  const long long pub = (long long)member * a.pmstride + (long long)blockIdx.x * NL * NX;
+  const uint32_t so = (uint32_t)cc * NXH;   // spectral row offset
+  const uint32_t po = (uint32_t)cc * NX;    // physical row offset
+  const uint32_t so_st = (uint32_t)c * NXH;
+  const uint32_t po_st = (uint32_t)c * NX;
   const cpx<T> zero = {(T)0, (T)0};
 
   if constexpr (MODE == ROW_C2R) {
-    cpx<T> z[R];
+    cpx<T> z[R], zm[R];
     const T s = (T)a.scale;
+    load_row_pairs<T, M, R, Lay>(a.in[0] + sub + so, z, zm, lds, c, j);
 #pragma unroll
     for (int q = 0; q < R; ++q) {
       const int k = j + q * P;
-      if (valid) {
-        const cpx<T> Xk = a.in[0][sbase + k];
-        const cpx<T> Xm = a.in[0][sbase + (M - k)];
-        z[q] = c2r_pre(Xk, Xm, a.tw[k * a.twstride], s, k == 0);
-      } else {
-        z[q] = zero;
-      }
+      z[q] = c2r_pre(z[q], zm[q], a.tw[k * a.twstride], s, k == 0);
     }
     line_fft<T, M, R, +1, Lay>(z, lds, c, j, a.tw, 2 * a.twstride);
     if (valid) {
+      cpx<T>* out = reinterpret_cast<cpx<T>*>(a.rout + pub);
 #pragma unroll
-      for (int q = 0; q < R; ++q) {
-        const int m = j + q * P;
-        using T2 = cpx<T>;
-        *reinterpret_cast<T2*>(a.rout + pbase + 2 * m) = z[q];
-      }
+      for (int q = 0; q < R; ++q) out[(po_st >> 1) + j + q * P] = z[q];
     }
     return;
   } else {
     cpx<T> f[R];
     if constexpr (MODE == ROW_R2C) {
+      const cpx<T>* rin = reinterpret_cast<const cpx<T>*>(a.rin + pub);
 #pragma unroll
-      for (int q = 0; q < R; ++q) {
-        const int m = j + q * P;
-        f[q] = valid ? *reinterpret_cast<const cpx<T>*>(a.rin + pbase + 2 * m) : zero;
-      }
+      for (int q = 0; q < R; ++q) f[q] = rin[(po >> 1) + j + q * P];
     } else {
       // ROW_ADV: f = ((-u cx) - v cy) - w cz, cx = C2R(i kr Y0), cy = C2R(Y1), cz = C2R(Y2)
       const T s = (T)a.scale;
-#pragma unroll
+#pragma unroll 1
       for (int d = 0; d < NIN; ++d) {
-        cpx<T> z[R];
+        cpx<T> z[R], zm[R];
+        load_row_pairs<T, M, R, Lay>(a.in[d] + sub + so, z, zm, lds, c, j);
 #pragma unroll
         for (int q = 0; q < R; ++q) {
           const int k = j + q * P;
-          if (valid) {
-            cpx<T> Xk = a.in[d][sbase + k];
-            cpx<T> Xm = a.in[d][sbase + (M - k)];
-            if (d == 0) {
-              Xk = times_ik(Xk, (T)(k * a.kscale_x));
-              Xm = times_ik(Xm, (T)((M - k) * a.kscale_x));
-            }
-            z[q] = c2r_pre(Xk, Xm, a.tw[k * a.twstride], s, k == 0);
-          } else {
-            z[q] = zero;
+          cpx<T> Xk = z[q];
+          cpx<T> Xm = zm[q];
+          if (d == 0) {
+            Xk = times_ik(Xk, (T)(k * a.kscale_x));
+            Xm = times_ik(Xm, (T)((M - k) * a.kscale_x));
           }
+          z[q] = c2r_pre(Xk, Xm, a.tw[k * a.twstride], s, k == 0);
         }
         line_fft<T, M, R, +1, Lay>(z, lds, c, j, a.tw, 2 * a.twstride);
+        const cpx<T>* fl = reinterpret_cast<const cpx<T>*>(a.flow[d] + pub);
 #pragma unroll
         for (int q = 0; q < R; ++q) {
-          const int m = j + q * P;
-          const cpx<T> uu = valid ? *reinterpret_cast<const cpx<T>*>(a.flow[d] + pbase + 2 * m) : zero;
+          const cpx<T> uu = fl[(po >> 1) + j + q * P];
           if (d == 0) {
             f[q] = {-uu.re * z[q].re, -uu.im * z[q].im};
           } else {
@@ -298,6 +367,7 @@ __global__ __launch_bounds__(NL*((NX / 2) / R)) void row_kernel(RowArgs<T> a) {
     for (int q = 0; q < R; ++q) lds[Lay::idx(c, j + q * P)] = f[q];
     __syncthreads();
     if (valid) {
+      const int iky = (int)(row % a.ny), ikz = (int)(row / a.ny);
 #pragma unroll
       for (int q = 0; q < R; ++q) {
         const int k = j + q * P;
@@ -308,31 +378,24 @@ __global__ __launch_bounds__(NL*((NX / 2) / R)) void row_kernel(RowArgs<T> a) {
           const cpx<T> Zm = lds[Lay::idx(c, M - k)];
           X = r2c_post(f[q], Zm, a.tw[k * a.twstride]);
         }
-        if constexpr (MODE == ROW_ADV) {
-          if (a.st.kind != EPI_STORE) {
-            const int iky = (int)(row % a.ny), ikz = (int)(row / a.ny);
-            stage_epilogue(a.st, sbase + k, member, k, iky, ikz, X);
-          } else {
-            a.out[sbase + k] = X;
-          }
-        } else {
-          a.out[sbase + k] = X;
+        // only 1D problems finish a stage in the x kernel (x is their last forward axis)
+        constexpr bool kEpi = MODE == ROW_ADV && NIN == 1;
+        bool epi = false;
+        if constexpr (kEpi) epi = a.st.kind != EPI_STORE;
+        if constexpr (kEpi) {
+          if (epi) stage_epilogue(a.st, sub, so_st + k, member, k, iky, ikz, X);
         }
+        if (!epi) (a.out + sub)[so_st + k] = X;
         if (k == 0) {
           const cpx<T> XM = {f[q].re - f[q].im, (T)0};
-          if constexpr (MODE == ROW_ADV) {
-            if (a.st.kind != EPI_STORE) {
-              const int iky = (int)(row % a.ny), ikz = (int)(row / a.ny);
-              stage_epilogue(a.st, sbase + M, member, M, iky, ikz, XM);
-            } else {
-              a.out[sbase + M] = XM;
-            }
-          } else {
-            a.out[sbase + M] = XM;
+          if constexpr (kEpi) {
+            if (epi) stage_epilogue(a.st, sub, so_st + M, member, M, iky, ikz, XM);
           }
+          if (!epi) (a.out + sub)[so_st + M] = XM;
         }
       }
     }
+    (void)zero;
   }
 }
 

@@ -35,8 +35,8 @@ constexpr int row_nin(int k) { return k == K_ROW_ADV2 ? 2 : k == K_ROW_ADV3 ? 3 
 template <typename T, int N, int R, int TX, int KIND>
 const void* colp() {
   constexpr int P = N / R;
-  constexpr bool ok = R <= N && (R >= 8 || N <= 16) && TX * P >= 16 && TX * P <= 512 &&
-                      (long long)N * TX * (long long)sizeof(cpx<T>) <= 65536;
+  constexpr bool ok = R <= N && (R >= 8 || N <= 16) && TX * P >= 16 && TX * P <= 1024 &&
+                      (long long)N * TX * (long long)sizeof(cpx<T>) <= 163840;
   if constexpr (ok) {
     return reinterpret_cast<const void*>(&col_kernel<T, N, R, TX, col_mode(KIND), col_dir(KIND)>);
   } else {
@@ -48,7 +48,7 @@ template <typename T, int NX, int R, int NL, int KIND>
 const void* rowp() {
   constexpr int M = NX / 2;
   constexpr int P = M / R;
-  constexpr bool ok = R <= M && (R >= 8 || M <= 16) && NL * P >= 16 && NL * P <= 512 &&
+  constexpr bool ok = R <= M && (R >= 4) && NL * P >= 16 && NL * P <= 512 &&
                       (long long)NL * (M + 1) * (long long)sizeof(cpx<T>) <= 65536;
   if constexpr (ok) {
     return reinterpret_cast<const void*>(&row_kernel<T, NX, R, NL, row_mode(KIND), row_nin(KIND)>);
@@ -66,6 +66,7 @@ const void* by_w(int W) {
       case 8: return colp<T, N, R, 8, KIND>();
       case 16: return colp<T, N, R, 16, KIND>();
       case 32: return colp<T, N, R, 32, KIND>();
+      case 64: return colp<T, N, R, 64, KIND>();
       default: return nullptr;
     }
   } else {
