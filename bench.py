@@ -39,6 +39,8 @@ def kernel_bytes(name: str, nS: int, nR: int, cb: int, rb: int, stage: int):
     read or write once.  cb/rb = bytes per complex/real element."""
     S, R = nS * cb, nR * rb
     fam = name.split(" [")[0].split("_", 1)[1] if name.startswith("stage") else name.split(" [")[0]
+    if fam.startswith("prologue_"):
+        return 3 * S
     if fam == "z_inverse_deriv":
         return 3 * S                     # read c, write IFFT_z c, IFFT_z(i m c)
     if fam == "y_inverse_deriv2":
@@ -52,10 +54,16 @@ def kernel_bytes(name: str, nS: int, nR: int, cb: int, rb: int, stage: int):
         return 2 * S + R + (2 * S if stage == 0 else 3 * S)
     if fam == "y_forward":
         return 2 * S
+    state = {0: 3 * S, 1: 5 * S, 2: 5 * S, 3: 4 * S}[stage]
     if fam in ("z_forward_stage", "y_forward_stage"):
         # read F; RK4 stage 0: read sol, write stg+acc; 1,2: read sol,stg,acc, write stg,acc;
         # 3: read stg,acc,sol, write sol
-        return S + {0: 3 * S, 1: 5 * S, 2: 5 * S, 3: 4 * S}[stage]
+        return S + state
+    if fam in ("z_forward_stage_inverse", "y_forward_stage_inverse"):
+        # as above, then the next stage's first inverse pass from registers: write 2
+        return S + state + 2 * S
+    if fam in ("prologue_z_inverse_deriv", "prologue_y_inverse_deriv"):
+        return 3 * S
     return None
 
 
@@ -266,8 +274,10 @@ def main():
     forms = {
         "abc3d_256_fp64": dict(value=value, ms_per_step=1e3 * secs / args.steps, steps=args.steps,
                                kernel_ms_per_step=step_ms_kernels,
-                               pass_model_bytes_per_step=step_bytes,
-                               pass_model_GBs=step_bytes / (secs / args.steps) / 1e9,
+                               schedule_bytes_per_step=step_bytes,
+                               schedule_GBs=step_bytes / (secs / args.steps) / 1e9,
+                               survey_pass_model_GBs=716.8 * value / 1e9,
+                               survey_pass_model_frac=716.8 * value / 1e9 / HBM_PEAK_GBS,
                                mass_drift_rel_to_l1=abs(m1 - m0) / l1,
                                variance_ratio=v1 / v0),
         "abc3d_256_mixed": dict(value=gps(npts, args.steps, secs32),

@@ -29,7 +29,7 @@ COMMON = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-I", CSRC, "-I", INCLUDE, "-Wno-unused-command-line-argument",
 ]
-KINDS = [0, 1, 2, 3, 4, 5, 10, 11, 12, 13, 14]
+KINDS = [0, 1, 2, 3, 4, 5, 6, 10, 11, 12, 13, 14]
 PRECS = [("double", "f64"), ("float", "f32")]
 HEADERS = ["ptf_device.hpp", "ptf_kernels.hpp"]
 API_HEADERS = HEADERS + ["ptf_setup.hpp"]

@@ -48,12 +48,14 @@ static int fail(int code, const std::string& msg) {
 // ---------------------------------------------------------------------------
 enum {
   K_COL_FWD = 0, K_COL_INV = 1, K_COL_D1 = 2, K_COL_D2 = 3, K_COL_STAGE = 4, K_COL_FLOWSYN = 5,
+  K_COL_STAGE_D1 = 6,
   K_ROW_C2R = 10, K_ROW_R2C = 11, K_ROW_ADV1 = 12, K_ROW_ADV2 = 13, K_ROW_ADV3 = 14
 };
 
 #define PTF_DECL(t, k) extern "C" const void* ptf_reg_##t##_##k(int, int, int);
 #define PTF_DECL_ALL(t) \
   PTF_DECL(t, 0) PTF_DECL(t, 1) PTF_DECL(t, 2) PTF_DECL(t, 3) PTF_DECL(t, 4) PTF_DECL(t, 5) \
+  PTF_DECL(t, 6) \
   PTF_DECL(t, 10) PTF_DECL(t, 11) PTF_DECL(t, 12) PTF_DECL(t, 13) PTF_DECL(t, 14)
 PTF_DECL_ALL(f64)
 PTF_DECL_ALL(f32)
@@ -63,7 +65,7 @@ typedef const void* (*RegFn)(int, int, int);
 static RegFn reg_fn(bool dbl, int kind) {
 #define PTF_CASE(k) case k: return dbl ? ptf_reg_f64_##k : ptf_reg_f32_##k;
   switch (kind) {
-    PTF_CASE(0) PTF_CASE(1) PTF_CASE(2) PTF_CASE(3) PTF_CASE(4) PTF_CASE(5)
+    PTF_CASE(0) PTF_CASE(1) PTF_CASE(2) PTF_CASE(3) PTF_CASE(4) PTF_CASE(5) PTF_CASE(6)
     PTF_CASE(10) PTF_CASE(11) PTF_CASE(12) PTF_CASE(13) PTF_CASE(14)
     default: return nullptr;
   }
@@ -78,6 +80,7 @@ static const char* kind_name(int kind) {
     case K_COL_D2: return "col_inv_deriv2";
     case K_COL_STAGE: return "col_fwd_stage";
     case K_COL_FLOWSYN: return "col_flow_synth";
+    case K_COL_STAGE_D1: return "col_fwd_stage_inv";
     case K_ROW_C2R: return "row_c2r";
     case K_ROW_R2C: return "row_r2c";
     case K_ROW_ADV1: return "row_advect1";
@@ -87,7 +90,7 @@ static const char* kind_name(int kind) {
   return "?";
 }
 
-static const int kRadices[] = {16, 8, 4};
+static const int kRadices[] = {16, 8, 4};  // tried in this order
 static const int kColWidths[] = {16, 8, 32, 64, 4, 2};
 static const int kNumColWidths = 6;
 static const int kRowWidths[] = {8, 16, 4, 2, 1, 1};
@@ -102,14 +105,17 @@ struct Slot {
   int kind = 0;
   int N = 0;            // axis length (columns) or nx (rows)
   int R = 0, W = 0;     // radix, columns/rows per block
+  int occ = 0;          // 0: default register budget; 3: 3 waves/SIMD
   const void* fn = nullptr;
   long long lines = 0;  // ncols (columns) or nrows (rows)
   int other = 1;        // grid.y for columns
   int members = 1;
   std::vector<unsigned char> args;  // ColArgs<T> or RowArgs<T>
   bool writes_state = false;        // launch modifies sol (timing must restore it)
+  int brick = 0;                    // log2 brick: columns along a bricked line / bricked rows
 
   bool is_col() const { return kind < 10; }
+  bool has_occ() const { return kind == 3 || kind == 6 || kind == 13 || kind == 14; }
   int threads() const { return is_col() ? W * (N / R) : W * ((N / 2) / R); }
   dim3 grid() const {
     if (is_col()) return dim3((unsigned)((lines + W - 1) / W), (unsigned)other, (unsigned)members);
@@ -117,13 +123,20 @@ struct Slot {
   }
 };
 
-static bool choose_variant(Slot& s, int R, int W) {
+static bool choose_variant(Slot& s, int R, int W, int occ = 0) {
   RegFn f = reg_fn(s.dbl, s.kind);
   if (!f) return false;
-  const void* p = f(s.N, R, W);
+  if (occ && !s.has_occ()) return false;
+  if (s.brick > 0) {
+    // a bricked line needs whole bricks per thread group; bricked rows whole bricks per block
+    if (s.is_col() && (R > s.N || (s.N / R) % (1 << s.brick) != 0)) return false;
+    if (!s.is_col() && W % (1 << s.brick) != 0) return false;
+  }
+  const void* p = f(s.N, R, W + 1000 * occ);
   if (!p) return false;
   s.R = R;
   s.W = W;
+  s.occ = occ;
   s.fn = p;
   return true;
 }
@@ -247,6 +260,7 @@ static bool is_pow2(long long n) { return n > 0 && (n & (n - 1)) == 0; }
 struct Group {
   int ndim = 1, nx = 1, ny = 1, nz = 1, nxh = 1;
   int B = 1;
+  int lb = 0;  // the last axis is stored in bricks of 2^lb planes (2D: y, 3D: z)
   bool dbl = true;
   size_t esz = 8;
   int stepper = PTF_RK4;
@@ -257,10 +271,17 @@ struct Group {
   double Ls[3] = {0, 0, 0}, ks[3] = {0, 0, 0};
   std::vector<OpParams> ops;
 
+  std::vector<double> L_host;  // L in the public layout, all members
   DevBuf d_ops, sol, sol_lo, stg, stg2, acc, n1, n2, w[3], flow[3], d_L, etd[6], tw[3], psih0, omega;
   DevBuf phys, spec, red, backup;  // set_c/updatevars/calcN staging, reductions, timing backup
+  DevBuf fs[2];                    // spectral-flow synthesis scratch
+  std::vector<Slot> prologue;      // first inverse pass of a step when w0/w1 are not primed
   std::vector<Slot> stages[4];
   std::vector<Slot> flowsyn;
+  // w0/w1 hold the first inverse pass of the current state (the fused last
+  // kernel of each step leaves them so); anything else that writes sol or the
+  // transform scratch clears this
+  bool primed = false;
   // plan slots (B = 1 view, pointers set per call)
   Slot p_r2c, p_c2r, p_yf, p_yi, p_zf, p_zi;
   hipStream_t stream = nullptr;
@@ -321,16 +342,28 @@ static ColArgs<T> col_base(const Group& g, int axis) {
   std::memset(&a, 0, sizeof(a));
   a.tw = (const cpx<T>*)g.tw[axis].p;
   a.twstride = 1;
-  a.ncols = g.nxh;
   a.mstride = g.nS;
   a.kscale = g.ks[axis];
   a.axis = axis;
-  if (axis == 1) {
-    a.stride = g.nxh;
-    a.ostride = (long long)g.nxh * g.ny;
+  const long long B = 1LL << g.lb;
+  if (g.ndim == 3 && axis == 1) {
+    // y pass of a 3D field: columns are (kx, kz_lo) pairs, blockIdx.y = kz_hi
+    a.ncols = (int)(g.nxh * B);
+    a.stride = g.nxh * B;
+    a.ostride = (long long)g.nxh * g.ny * B;
+    a.lbc = g.lb;
+  } else if (g.ndim == 3) {
+    // z pass: the line runs along the bricked axis, blockIdx.y = ky
+    a.ncols = g.nxh;
+    a.stride = (long long)g.nxh * g.ny * B;
+    a.ostride = g.nxh * B;
+    a.lb = g.lb;
   } else {
-    a.stride = (long long)g.nxh * g.ny;
-    a.ostride = g.nxh;
+    // y pass of a 2D field: the line runs along the bricked axis
+    a.ncols = g.nxh;
+    a.stride = g.nxh * B;
+    a.ostride = 0;
+    a.lb = g.lb;
   }
   a.kscale_x = g.ks[0];
   return a;
@@ -348,6 +381,7 @@ static RowArgs<T> row_base(const Group& g) {
   a.scale = 1.0 / ((double)g.nx * g.ny * g.nz);
   a.kscale_x = g.ks[0];
   a.ny = g.ny;
+  a.lb = g.lb;
   return a;
 }
 
@@ -357,12 +391,25 @@ static Slot make_slot(const Group& g, int kind, int axis, int members) {
   s.kind = kind;
   s.members = members;
   if (kind < 10) {
-    s.N = axis == 1 ? g.ny : g.nz;
-    s.lines = g.nxh;
-    s.other = axis == 1 ? g.nz : g.ny;
+    if (g.ndim == 3 && axis == 1) {
+      s.N = g.ny;
+      s.lines = (long long)g.nxh << g.lb;
+      s.other = g.nz >> g.lb;
+    } else if (g.ndim == 3) {
+      s.N = g.nz;
+      s.lines = g.nxh;
+      s.other = g.ny;
+      s.brick = g.lb;
+    } else {
+      s.N = g.ny;
+      s.lines = g.nxh;
+      s.other = 1;
+      s.brick = g.lb;
+    }
   } else {
     s.N = g.nx;
     s.lines = (long long)g.ny * g.nz;
+    s.brick = g.lb;
   }
   return s;
 }
@@ -396,9 +443,12 @@ static StageArgs<T> stage_args(const Group& g, int stage) {
 
 // Launch list of one calcN! evaluation of `in`, ending in `st` (a stage
 // epilogue or a plain store to `out`).
+// With fused = true the first inverse pass is left out (the previous stage's
+// last kernel produced w0/w1) and the last kernel also performs the first
+// inverse pass of the next stage's input (K_COL_STAGE_D1).
 template <typename T>
 static int build_calcN(Group& g, const cpx<T>* in, const StageArgs<T>& st, cpx<T>* out,
-                       std::vector<Slot>& list, const std::string& prefix) {
+                       std::vector<Slot>& list, const std::string& prefix, bool fused = false) {
   cpx<T>* w0 = (cpx<T>*)g.w[0].p;
   cpx<T>* w1 = (cpx<T>*)g.w[1].p;
   cpx<T>* w2 = (cpx<T>*)g.w[2].p;
@@ -420,15 +470,17 @@ static int build_calcN(Group& g, const cpx<T>* in, const StageArgs<T>& st, cpx<T
     return PTF_OK;
   }
   if (g.ndim == 2) {
-    Slot s1 = make_slot(g, K_COL_D1, 1, g.B);
-    ColArgs<T> a1 = col_base<T>(g, 1);
-    a1.in0 = in;
-    a1.out0 = w0;
-    a1.out1 = w1;
-    set_args(s1, a1);
-    s1.name = prefix + "y_inverse_deriv";
-    CHK(pick_default(s1));
-    list.push_back(s1);
+    if (!fused) {
+      Slot s1 = make_slot(g, K_COL_D1, 1, g.B);
+      ColArgs<T> a1 = col_base<T>(g, 1);
+      a1.in0 = in;
+      a1.out0 = w0;
+      a1.out1 = w1;
+      set_args(s1, a1);
+      s1.name = prefix + "y_inverse_deriv";
+      CHK(pick_default(s1));
+      list.push_back(s1);
+    }
 
     Slot s2 = make_slot(g, K_ROW_ADV2, 0, g.B);
     RowArgs<T> a2 = row_base<T>(g);
@@ -443,28 +495,31 @@ static int build_calcN(Group& g, const cpx<T>* in, const StageArgs<T>& st, cpx<T
     CHK(pick_default(s2));
     list.push_back(s2);
 
-    Slot s3 = make_slot(g, K_COL_STAGE, 1, g.B);
+    Slot s3 = make_slot(g, fused ? K_COL_STAGE_D1 : K_COL_STAGE, 1, g.B);
     ColArgs<T> a3 = col_base<T>(g, 1);
     a3.in0 = w0;
-    a3.out0 = out;
+    a3.out0 = fused ? w0 : out;
+    a3.out1 = fused ? w1 : nullptr;
     a3.st = st;
     set_args(s3, a3);
-    s3.name = prefix + "y_forward_stage";
+    s3.name = prefix + (fused ? "y_forward_stage_inverse" : "y_forward_stage");
     s3.writes_state = st.kind != EPI_STORE && st.stage == 3;
     CHK(pick_default(s3));
     list.push_back(s3);
     return PTF_OK;
   }
   // 3D
-  Slot s1 = make_slot(g, K_COL_D1, 2, g.B);
-  ColArgs<T> a1 = col_base<T>(g, 2);
-  a1.in0 = in;
-  a1.out0 = w0;
-  a1.out1 = w1;
-  set_args(s1, a1);
-  s1.name = prefix + "z_inverse_deriv";
-  CHK(pick_default(s1));
-  list.push_back(s1);
+  if (!fused) {
+    Slot s1 = make_slot(g, K_COL_D1, 2, g.B);
+    ColArgs<T> a1 = col_base<T>(g, 2);
+    a1.in0 = in;
+    a1.out0 = w0;
+    a1.out1 = w1;
+    set_args(s1, a1);
+    s1.name = prefix + "z_inverse_deriv";
+    CHK(pick_default(s1));
+    list.push_back(s1);
+  }
 
   Slot s2 = make_slot(g, K_COL_D2, 1, g.B);
   ColArgs<T> a2 = col_base<T>(g, 1);
@@ -502,13 +557,14 @@ static int build_calcN(Group& g, const cpx<T>* in, const StageArgs<T>& st, cpx<T
   CHK(pick_default(s4));
   list.push_back(s4);
 
-  Slot s5 = make_slot(g, K_COL_STAGE, 2, g.B);
+  Slot s5 = make_slot(g, fused ? K_COL_STAGE_D1 : K_COL_STAGE, 2, g.B);
   ColArgs<T> a5 = col_base<T>(g, 2);
   a5.in0 = w0;
-  a5.out0 = out;
+  a5.out0 = fused ? w0 : out;
+  a5.out1 = fused ? w1 : nullptr;
   a5.st = st;
   set_args(s5, a5);
-  s5.name = prefix + "z_forward_stage";
+  s5.name = prefix + (fused ? "z_forward_stage_inverse" : "z_forward_stage");
   s5.writes_state = st.kind != EPI_STORE && st.stage == 3;
   CHK(pick_default(s5));
   list.push_back(s5);
@@ -526,16 +582,31 @@ static int build_group_slots(Group& g) {
       in = (const cpx<T>*)(s == 0 ? g.sol.p : s == 1 ? g.stg.p : g.stg2.p);
     }
     StageArgs<T> st = stage_args<T>(g, s);
-    CHK(build_calcN<T>(g, in, st, nullptr, g.stages[s], "stage" + std::to_string(s + 1) + "_"));
+    CHK(build_calcN<T>(g, in, st, nullptr, g.stages[s], "stage" + std::to_string(s + 1) + "_",
+                       g.ndim >= 2));
   }
+  g.prologue.clear();
+  if (g.ndim >= 2) {
+    const int axis = g.ndim == 3 ? 2 : 1;
+    Slot s1 = make_slot(g, K_COL_D1, axis, g.B);
+    ColArgs<T> a1 = col_base<T>(g, axis);
+    a1.in0 = (const cpx<T>*)g.sol.p;
+    a1.out0 = (cpx<T>*)g.w[0].p;
+    a1.out1 = (cpx<T>*)g.w[1].p;
+    set_args(s1, a1);
+    s1.name = axis == 2 ? "prologue_z_inverse_deriv" : "prologue_y_inverse_deriv";
+    CHK(pick_default(s1));
+    g.prologue.push_back(s1);
+  }
+  g.primed = false;
   g.flowsyn.clear();
   if (g.flow_kind == PTF_FLOW_SPECTRAL) {
     Slot s1 = make_slot(g, K_COL_FLOWSYN, 1, 1);
     ColArgs<T> a1 = col_base<T>(g, 1);
     a1.psih0 = (const cpx<double>*)g.psih0.p;
     a1.omega = (const double*)g.omega.p;
-    a1.out0 = (cpx<T>*)g.w[0].p;
-    a1.out1 = (cpx<T>*)g.w[1].p;
+    a1.out0 = (cpx<T>*)g.fs[0].p;
+    a1.out1 = (cpx<T>*)g.fs[1].p;
     set_args(s1, a1);
     s1.name = "flow_y_synthesis";
     CHK(pick_default(s1));
@@ -543,7 +614,7 @@ static int build_group_slots(Group& g) {
     for (int c = 0; c < 2; ++c) {
       Slot s2 = make_slot(g, K_ROW_C2R, 0, 1);
       RowArgs<T> a2 = row_base<T>(g);
-      a2.in[0] = (const cpx<T>*)g.w[c].p;
+      a2.in[0] = (const cpx<T>*)g.fs[c].p;
       a2.rout = (T*)g.flow[c].p;
       set_args(s2, a2);
       s2.name = c == 0 ? "flow_x_c2r_u" : "flow_x_c2r_v";
@@ -594,7 +665,75 @@ static int upload_twiddles(Group& g) {
   return PTF_OK;
 }
 
-static int group_init(Group& g, const ptf_desc* descs, int count) {
+// ---------------------------------------------------------------------------
+// public layout <-> brick layout (host side)
+// A member's rows are (l, m) in the public order r = l*mid + m (l = last-axis
+// index, m = middle index; mid = ny in 3D, 1 in 2D).  In the brick layout row
+// (l, m) becomes b = ((l >> lb) mid + m) 2^lb + (l mod 2^lb) and element k of
+// row b sits at ((b >> lb) rowlen + k) 2^lb + (b mod 2^lb).
+// ---------------------------------------------------------------------------
+static size_t brick_index(const Group& g, size_t r, size_t k, size_t rowlen) {
+  if (!g.lb) return r * rowlen + k;
+  const size_t mid = g.ndim == 3 ? (size_t)g.ny : 1;
+  const size_t l = r / mid, m = r % mid;
+  const size_t B = (size_t)1 << g.lb, mask = B - 1;
+  const size_t b = ((l >> g.lb) * mid + m) * B + (l & mask);
+  return ((b >> g.lb) * rowlen + k) * B + (b & mask);
+}
+
+static size_t member_rows(const Group& g) { return (size_t)g.ny * g.nz; }
+
+// w = doubles per element (2: complex values or physical x-pairs; 1: reals)
+static void to_brick(const Group& g, const double* src, double* dst, size_t rowlen, int w) {
+  const size_t rows = member_rows(g);
+  for (size_t r = 0; r < rows; ++r)
+    for (size_t k = 0; k < rowlen; ++k) {
+      const size_t d = brick_index(g, r, k, rowlen);
+      for (int e = 0; e < w; ++e) dst[d * w + e] = src[(r * rowlen + k) * w + e];
+    }
+}
+
+static void from_brick(const Group& g, const double* src, double* dst, size_t rowlen, int w) {
+  const size_t rows = member_rows(g);
+  for (size_t r = 0; r < rows; ++r)
+    for (size_t k = 0; k < rowlen; ++k) {
+      const size_t d = brick_index(g, r, k, rowlen);
+      for (int e = 0; e < w; ++e) dst[(r * rowlen + k) * w + e] = src[d * w + e];
+    }
+}
+
+// host FP64 arrays of one member in the public layout -> the group's layout (and back)
+static const double* phys_in(const Group& g, const double* src, std::vector<double>& tmp) {
+  if (!g.lb) return src;
+  tmp.resize((size_t)g.nR);
+  to_brick(g, src, tmp.data(), (size_t)g.nx / 2, 2);
+  return tmp.data();
+}
+static const double* spec_in(const Group& g, const double* src, std::vector<double>& tmp, int w) {
+  if (!g.lb) return src;
+  tmp.resize((size_t)g.nS * w);
+  to_brick(g, src, tmp.data(), (size_t)g.nxh, w);
+  return tmp.data();
+}
+
+static int group_init(Group& g, const ptf_desc* descs, int count, bool allow_brick = true);
+
+static int group_init_once(Group& g, const ptf_desc* descs, int count, int lb);
+
+static int group_init(Group& g, const ptf_desc* descs, int count, bool allow_brick) {
+  const ptf_desc& d = descs[0];
+  int lb = 0;
+  if (allow_brick && d.ndim >= 2) {
+    const long long nlast = d.ndim == 3 ? d.n[2] : d.n[1];
+    if (nlast >= 64) lb = 3;
+  }
+  int rc = group_init_once(g, descs, count, lb);
+  if (rc == PTF_ERR_UNSUPPORTED && lb > 0) rc = group_init_once(g, descs, count, 0);
+  return rc;
+}
+
+static int group_init_once(Group& g, const ptf_desc* descs, int count, int lb) {
+  g.lb = lb;
   const ptf_desc& d = descs[0];
   g.ndim = d.ndim;
   g.nx = (int)d.n[0];
@@ -646,6 +785,8 @@ static int group_init(Group& g, const ptf_desc* descs, int count) {
   CHK(dalloc(g.spec, (size_t)g.nS * cs));
   CHK(dalloc(g.red, 2 * sizeof(double) * 1024));
   if (g.flow_kind == PTF_FLOW_SPECTRAL) {
+    CHK(dalloc(g.fs[0], (size_t)g.nS * cs));
+    CHK(dalloc(g.fs[1], (size_t)g.nS * cs));
     CHK(dalloc(g.psih0, (size_t)g.nS * 2 * sizeof(double)));
     CHK(dalloc(g.omega, (size_t)g.nS * sizeof(double)));
     HIPCHK(hipMemset(g.psih0.p, 0, (size_t)g.nS * 2 * sizeof(double)));
@@ -680,6 +821,13 @@ static int group_init(Group& g, const ptf_desc* descs, int count) {
             }
             L[(size_t)b * g.nS + kx + (size_t)g.nxh * (ky + (size_t)g.ny * kz)] = v;
           }
+    }
+    g.L_host = L;
+    if (g.lb) {
+      std::vector<double> Lb(L.size());
+      for (int b = 0; b < g.B; ++b)
+        to_brick(g, L.data() + (size_t)b * g.nS, Lb.data() + (size_t)b * g.nS, (size_t)g.nxh, 1);
+      L.swap(Lb);
     }
     CHK(dalloc(g.d_L, L.size() * sizeof(double)));
     HIPCHK(hipMemcpy(g.d_L.p, L.data(), L.size() * sizeof(double), hipMemcpyHostToDevice));
@@ -793,8 +941,19 @@ static int group_step_launches(Group& g, hipStream_t st, double t) {
     else args_of<ColArgs<float>>(s).t = t;
     for (auto& fs : g.flowsyn) CHK(launch(fs, st));
   }
+  if (!g.primed)
+    for (auto& sl : g.prologue) CHK(launch(sl, st));
   for (int s = 0; s < 4; ++s)
     for (auto& sl : g.stages[s]) CHK(launch(sl, st));
+  g.primed = true;
+  return PTF_OK;
+}
+
+// launch the prologue if needed so that a captured step can assume primed scratch
+static int group_prime(Group& g, hipStream_t st) {
+  if (!g.primed)
+    for (auto& sl : g.prologue) CHK(launch(sl, st));
+  g.primed = true;
   return PTF_OK;
 }
 
@@ -809,6 +968,7 @@ static std::vector<Slot*> step_slots(Group& g) {
 
 static int time_slot(Group& g, Slot& s, int reps, hipStream_t st, double* avg_ms) {
   if (reps < 1) reps = 1;
+  g.primed = false;  // trial launches overwrite the transform scratch
   const size_t S = (size_t)g.nS * g.B * 2 * g.esz;
   if (s.writes_state) {
     if (!g.backup.p) CHK(dalloc(g.backup, 2 * S));
@@ -849,29 +1009,32 @@ static int autotune_group(Group& g, hipStream_t st) {
     done.push_back(key);
     const int* widths = s->is_col() ? kColWidths : kRowWidths;
     const int nw = s->is_col() ? kNumColWidths : kNumRowWidths;
-    int bestR = s->R, bestW = s->W;
+    int bestR = s->R, bestW = s->W, bestOcc = s->occ;
     double best = 1e30;
     const bool verbose = std::getenv("PTF_AUTOTUNE_VERBOSE") != nullptr;
-    for (int R : kRadices) {
-      for (int wi = 0; wi < nw; ++wi) {
-        Slot trial = *s;
-        if (!choose_variant(trial, R, widths[wi])) continue;
-        double ms = 0;
-        CHK(time_slot(g, trial, 5, st, &ms));
-        if (verbose)
-          std::fprintf(stderr, "autotune %s N=%d R=%d W=%d: %.4f ms\n", kind_name(trial.kind),
-                       trial.N, R, widths[wi], ms);
-        if (ms < best) {
-          best = ms;
-          bestR = R;
-          bestW = widths[wi];
+    for (int occ : {0, 3}) {
+      for (int R : kRadices) {
+        for (int wi = 0; wi < nw; ++wi) {
+          Slot trial = *s;
+          if (!choose_variant(trial, R, widths[wi], occ)) continue;
+          double ms = 0;
+          CHK(time_slot(g, trial, 5, st, &ms));
+          if (verbose)
+            std::fprintf(stderr, "autotune %s N=%d R=%d W=%d occ=%d: %.4f ms\n",
+                         kind_name(trial.kind), trial.N, R, widths[wi], occ, ms);
+          if (ms < best) {
+            best = ms;
+            bestR = R;
+            bestW = widths[wi];
+            bestOcc = occ;
+          }
         }
       }
     }
     for (Slot* o : all) {
       const std::string k2 = std::to_string(o->kind) + ":" + std::to_string(o->N) + ":" +
                              std::to_string(o->lines) + ":" + std::to_string(o->members);
-      if (k2 == key) choose_variant(*o, bestR, bestW);
+      if (k2 == key) choose_variant(*o, bestR, bestW, bestOcc);
     }
   }
   return PTF_OK;
@@ -921,27 +1084,37 @@ struct ptf_ensemble {
 };
 
 // sol of one member as FP64 (hi + lo in the mixed form)
-static int get_member_sol(Group& g, int member, double* out, hipStream_t st) {
+static int get_member_sol(Group& g, int member, double* out_pub, hipStream_t st) {
   const size_t n = (size_t)g.nS * 2;
+  std::vector<double> lay;
+  double* out = out_pub;
+  if (g.lb) {
+    lay.resize(n);
+    out = lay.data();
+  }
   if (g.dbl) {
     HIPCHK(hipMemcpyAsync(out, (const char*)g.sol.p + (size_t)member * n * 8, n * 8,
                           hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
-    return PTF_OK;
+  } else {
+    std::vector<float> hi(n), lo(n);
+    HIPCHK(hipMemcpyAsync(hi.data(), (const char*)g.sol.p + (size_t)member * n * 4, n * 4,
+                          hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(lo.data(), (const char*)g.sol_lo.p + (size_t)member * n * 4, n * 4,
+                          hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    for (size_t i = 0; i < n; ++i) out[i] = (double)hi[i] + (double)lo[i];
   }
-  std::vector<float> hi(n), lo(n);
-  HIPCHK(hipMemcpyAsync(hi.data(), (const char*)g.sol.p + (size_t)member * n * 4, n * 4,
-                        hipMemcpyDeviceToHost, st));
-  HIPCHK(hipMemcpyAsync(lo.data(), (const char*)g.sol_lo.p + (size_t)member * n * 4, n * 4,
-                        hipMemcpyDeviceToHost, st));
-  HIPCHK(hipStreamSynchronize(st));
-  for (size_t i = 0; i < n; ++i) out[i] = (double)hi[i] + (double)lo[i];
+  if (g.lb) from_brick(g, out, out_pub, (size_t)g.nxh, 2);
   return PTF_OK;
 }
 
 // set sol of one member from FP64 (split into hi + lo in the mixed form)
-static int set_member_sol(Group& g, int member, const double* in, hipStream_t st) {
+static int set_member_sol(Group& g, int member, const double* in_pub, hipStream_t st) {
   const size_t n = (size_t)g.nS * 2;
+  g.primed = false;
+  std::vector<double> tmp;
+  const double* in = spec_in(g, in_pub, tmp, 2);
   if (g.dbl) {
     HIPCHK(hipMemcpyAsync((char*)g.sol.p + (size_t)member * n * 8, in, n * 8,
                           hipMemcpyHostToDevice, st));
@@ -1037,7 +1210,7 @@ int ptf_plan_create(int32_t ndim, const int64_t n[3], int32_t precision, ptf_pla
   d.dt = 1.0;
   CHK(validate_desc(&d));
   std::unique_ptr<ptf_plan> p(new ptf_plan());
-  CHK(group_init(p->g, &d, 1));
+  CHK(group_init(p->g, &d, 1, /*allow_brick=*/false));
   HIPCHK(hipStreamCreate(&p->stream));
   p->own_stream = true;
   *out = p.release();
@@ -1114,7 +1287,9 @@ int ptf_set_flow(ptf_problem* prob, const double* u, const double* v, const doub
   Group& g = prob->g;
   for (int i = 0; i < g.ndim; ++i) {
     if (!f[i]) return fail(PTF_ERR_INVALID, "flow component " + std::to_string(i) + " is NULL");
-    CHK(upload_real(g, g.flow[i].p, f[i], (size_t)g.nR, prob->stream));
+    std::vector<double> tmp;
+    CHK(upload_real(g, g.flow[i].p, phys_in(g, f[i], tmp), (size_t)g.nR, prob->stream));
+    HIPCHK(hipStreamSynchronize(prob->stream));
   }
   HIPCHK(hipStreamSynchronize(prob->stream));
   prob->flow_set = true;
@@ -1126,8 +1301,11 @@ int ptf_set_spectral_flow(ptf_problem* prob, const double* psih0, const double* 
   if (prob->d.flow_kind != PTF_FLOW_SPECTRAL)
     return fail(PTF_ERR_INVALID, "problem was not created with PTF_FLOW_SPECTRAL");
   Group& g = prob->g;
-  HIPCHK(hipMemcpy(g.psih0.p, psih0, (size_t)g.nS * 2 * sizeof(double), hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(g.omega.p, omega, (size_t)g.nS * sizeof(double), hipMemcpyHostToDevice));
+  std::vector<double> t1, t2;
+  HIPCHK(hipMemcpy(g.psih0.p, spec_in(g, psih0, t1, 2), (size_t)g.nS * 2 * sizeof(double),
+                   hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(g.omega.p, spec_in(g, omega, t2, 1), (size_t)g.nS * sizeof(double),
+                   hipMemcpyHostToDevice));
   prob->flow_set = true;
   return PTF_OK;
 }
@@ -1135,9 +1313,12 @@ int ptf_set_spectral_flow(ptf_problem* prob, const double* psih0, const double* 
 int ptf_set_c(ptf_problem* prob, const double* c) {
   if (!prob || !c) return fail(PTF_ERR_INVALID, "NULL argument");
   Group& g = prob->g;
-  CHK(upload_real(g, g.phys.p, c, (size_t)g.nR, prob->stream));
+  std::vector<double> tmp;
+  CHK(upload_real(g, g.phys.p, phys_in(g, c, tmp), (size_t)g.nR, prob->stream));
+  HIPCHK(hipStreamSynchronize(prob->stream));
   CHK(g_rfft(g, g.phys.p, g.sol.p, prob->stream));
   CHK(clear_member_lo(g, 0, prob->stream));
+  g.primed = false;
   HIPCHK(hipStreamSynchronize(prob->stream));
   return PTF_OK;
 }
@@ -1189,7 +1370,11 @@ int ptf_updatevars(ptf_problem* prob, double* c) {
   if (!prob || !c) return fail(PTF_ERR_INVALID, "NULL argument");
   Group& g = prob->g;
   CHK(g_irfft(g, g.sol.p, g.phys.p, prob->stream));
-  return download_real(g, c, g.phys.p, (size_t)g.nR, prob->stream);
+  if (!g.lb) return download_real(g, c, g.phys.p, (size_t)g.nR, prob->stream);
+  std::vector<double> tmp((size_t)g.nR);
+  CHK(download_real(g, tmp.data(), g.phys.p, (size_t)g.nR, prob->stream));
+  from_brick(g, tmp.data(), c, (size_t)g.nx / 2, 2);
+  return PTF_OK;
 }
 
 }  // extern "C"
@@ -1202,14 +1387,21 @@ static int calcN_impl(ptf_problem* prob, const double* sol, double* N) {
     args_of<ColArgs<T>>(g.flowsyn[0]).t = prob->t;
     for (auto& fs : g.flowsyn) CHK(launch(fs, st));
   }
-  CHK(upload_spec(g, g.stg.p, sol, st));
+  g.primed = false;  // the evaluation below uses the transform scratch
+  std::vector<double> tmp;
+  CHK(upload_spec(g, g.stg.p, spec_in(g, sol, tmp, 2), st));
+  HIPCHK(hipStreamSynchronize(st));
   StageArgs<T> none;
   std::memset(&none, 0, sizeof(none));
   none.kind = EPI_STORE;
   std::vector<Slot> list;
   CHK(build_calcN<T>(g, (const cpx<T>*)g.stg.p, none, (cpx<T>*)g.acc.p, list, "calcN_"));
   for (auto& s : list) CHK(launch(s, st));
-  return download_spec(g, N, g.acc.p, st);
+  if (!g.lb) return download_spec(g, N, g.acc.p, st);
+  std::vector<double> out((size_t)g.nS * 2);
+  CHK(download_spec(g, out.data(), g.acc.p, st));
+  from_brick(g, out.data(), N, (size_t)g.nxh, 2);
+  return PTF_OK;
 }
 
 extern "C" {
@@ -1221,7 +1413,7 @@ int ptf_calcN(ptf_problem* prob, const double* sol, double* N) {
 
 int ptf_get_L(ptf_problem* prob, double* L) {
   if (!prob || !L) return fail(PTF_ERR_INVALID, "NULL argument");
-  HIPCHK(hipMemcpy(L, prob->g.d_L.p, (size_t)prob->g.nS * sizeof(double), hipMemcpyDeviceToHost));
+  std::memcpy(L, prob->g.L_host.data(), (size_t)prob->g.nS * sizeof(double));
   return PTF_OK;
 }
 
@@ -1247,10 +1439,10 @@ static int diagnostics_group(Group& g, int member, hipStream_t st, double* mass,
     const int blocks = 1024;
     if (g.dbl)
       hipLaunchKernelGGL(variance_partial_kernel<double>, dim3(blocks), dim3(256), 0, st,
-                         (const cpx<double>*)base, g.nS, g.nxh, (double*)g.red.p);
+                         (const cpx<double>*)base, g.nS, g.nxh, g.lb, (double*)g.red.p);
     else
       hipLaunchKernelGGL(variance_partial_kernel<float>, dim3(blocks), dim3(256), 0, st,
-                         (const cpx<float>*)base, g.nS, g.nxh, (double*)g.red.p);
+                         (const cpx<float>*)base, g.nS, g.nxh, g.lb, (double*)g.red.p);
     HIPCHK(hipGetLastError());
     std::vector<double> part(2 * blocks);
     HIPCHK(hipMemcpyAsync(part.data(), g.red.p, part.size() * sizeof(double), hipMemcpyDeviceToHost, st));
@@ -1295,8 +1487,8 @@ int ptf_problem_num_slots(ptf_problem* prob, int32_t* n) {
 static int slot_name_into(Slot& s, char* buf, int32_t buflen) {
   if (!buf || buflen <= 0) return fail(PTF_ERR_INVALID, "bad buffer");
   char tmp[256];
-  std::snprintf(tmp, sizeof(tmp), "%s [%s N=%d R=%d W=%d %s]", s.name.c_str(), kind_name(s.kind),
-                s.N, s.R, s.W, s.dbl ? "f64" : "f32");
+  std::snprintf(tmp, sizeof(tmp), "%s [%s N=%d R=%d W=%d%s %s]", s.name.c_str(),
+                kind_name(s.kind), s.N, s.R, s.W, s.occ ? " occ=3" : "", s.dbl ? "f64" : "f32");
   std::strncpy(buf, tmp, (size_t)buflen - 1);
   buf[buflen - 1] = 0;
   return PTF_OK;
@@ -1414,7 +1606,9 @@ int ptf_ensemble_set_flow(ptf_ensemble* ens, int32_t member, const double* u, co
   for (int c = 0; c < g->ndim && c < 2; ++c) {
     if (!f[c]) return fail(PTF_ERR_INVALID, "NULL flow component");
     char* dst = (char*)g->flow[c].p + (size_t)i * g->nR * g->esz;
-    CHK(upload_real(*g, dst, f[c], (size_t)g->nR, ens->stream));
+    std::vector<double> tmp;
+    CHK(upload_real(*g, dst, phys_in(*g, f[c], tmp), (size_t)g->nR, ens->stream));
+    HIPCHK(hipStreamSynchronize(ens->stream));
   }
   HIPCHK(hipStreamSynchronize(ens->stream));
   return PTF_OK;
@@ -1425,10 +1619,13 @@ int ptf_ensemble_set_c(ptf_ensemble* ens, int32_t member, const double* c) {
   int i;
   CHK(member_of(ens, member, &g, &i));
   if (!c) return fail(PTF_ERR_INVALID, "c is NULL");
-  CHK(upload_real(*g, g->phys.p, c, (size_t)g->nR, ens->stream));
+  std::vector<double> tmp;
+  CHK(upload_real(*g, g->phys.p, phys_in(*g, c, tmp), (size_t)g->nR, ens->stream));
+  HIPCHK(hipStreamSynchronize(ens->stream));
   char* dst = (char*)g->sol.p + (size_t)i * g->nS * 2 * g->esz;
   CHK(g_rfft(*g, g->phys.p, dst, ens->stream));
   CHK(clear_member_lo(*g, i, ens->stream));
+  g->primed = false;
   HIPCHK(hipStreamSynchronize(ens->stream));
   return PTF_OK;
 }
@@ -1440,7 +1637,11 @@ int ptf_ensemble_get_c(ptf_ensemble* ens, int32_t member, double* c) {
   if (!c) return fail(PTF_ERR_INVALID, "c is NULL");
   char* src = (char*)g->sol.p + (size_t)i * g->nS * 2 * g->esz;
   CHK(g_irfft(*g, src, g->phys.p, ens->stream));
-  return download_real(*g, c, g->phys.p, (size_t)g->nR, ens->stream);
+  if (!g->lb) return download_real(*g, c, g->phys.p, (size_t)g->nR, ens->stream);
+  std::vector<double> tmp((size_t)g->nR);
+  CHK(download_real(*g, tmp.data(), g->phys.p, (size_t)g->nR, ens->stream));
+  from_brick(*g, tmp.data(), c, (size_t)g->nx / 2, 2);
+  return PTF_OK;
 }
 
 int ptf_ensemble_get_sol(ptf_ensemble* ens, int32_t member, double* sol) {
@@ -1468,6 +1669,7 @@ static int ensemble_record_step(ptf_ensemble* e) {
 int ptf_ensemble_step(ptf_ensemble* ens, int64_t nsteps, int32_t use_graph) {
   if (!ens) return fail(PTF_ERR_INVALID, "ensemble is NULL");
   if (nsteps < 0) return fail(PTF_ERR_INVALID, "nsteps must be >= 0");
+  for (auto& g : ens->groups) CHK(group_prime(*g, ens->stream));
   if (!use_graph) {
     for (int64_t i = 0; i < nsteps; ++i) CHK(ensemble_record_step(ens));
     HIPCHK(hipGetLastError());

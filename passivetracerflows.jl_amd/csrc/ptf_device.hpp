@@ -162,11 +162,11 @@ constexpr int ipow(int b, int e) { return e == 0 ? 1 : b * ipow(b, e - 1); }
 // LDS layouts: line c, element i
 template <int NL>
 struct LayInterleaved {  // lines interleaved: consecutive lanes = consecutive lines
-  static __device__ __forceinline__ int idx(int c, int i) { return i * NL + c; }
+  __device__ __forceinline__ int idx(int c, int i) const { return i * NL + c; }
 };
-template <int N, int PAD>
-struct LayRows {  // lines contiguous: consecutive lanes = consecutive elements
-  static __device__ __forceinline__ int idx(int c, int i) { return c * (N + PAD) + i; }
+struct LayStrided {      // runtime strides: line-major (rows contiguous) or element-major
+  int sc, si;
+  __device__ __forceinline__ int idx(int c, int i) const { return c * sc + i * si; }
 };
 
 // Line FFT of length N with radix R on P = N/R threads per line (see header).
@@ -174,7 +174,7 @@ struct LayRows {  // lines contiguous: consecutive lanes = consecutive elements
 // before its first LDS write; ends after its last LDS read (callers that write
 // LDS afterwards must barrier first).  Every thread of the block must call it.
 template <typename T, int N, int R, int DIR, class Lay>
-__device__ __forceinline__ void line_fft(cpx<T> (&a)[R], cpx<T>* lds, int c, int j,
+__device__ __forceinline__ void line_fft(cpx<T> (&a)[R], cpx<T>* lds, const Lay& lay, int c, int j,
                                          const cpx<T>* __restrict__ tw, int twstride) {
   constexpr int P = N / R;
   constexpr int K = uniform_passes(N, R);
@@ -194,14 +194,14 @@ __device__ __forceinline__ void line_fft(cpx<T> (&a)[R], cpx<T>* lds, int c, int
       dft<R0, DIR>(t);
       const int jj = j + b * P;
 #pragma unroll
-      for (int r = 0; r < R0; ++r) lds[Lay::idx(c, jj * R0 + r)] = t[r];
+      for (int r = 0; r < R0; ++r) lds[lay.idx(c, jj * R0 + r)] = t[r];
     }
     int Ns = R0;
 #pragma unroll
     for (int p = 1; p <= K; ++p) {
       __syncthreads();
 #pragma unroll
-      for (int r = 0; r < R; ++r) a[r] = lds[Lay::idx(c, j + r * P)];
+      for (int r = 0; r < R; ++r) a[r] = lds[lay.idx(c, j + r * P)];
       const int m = j % Ns;
       const int step = (N / (Ns * R)) * twstride;
 #pragma unroll
@@ -215,7 +215,7 @@ __device__ __forceinline__ void line_fft(cpx<T> (&a)[R], cpx<T>* lds, int c, int
         __syncthreads();
         const int base = (j / Ns) * Ns * R + m;
 #pragma unroll
-        for (int r = 0; r < R; ++r) lds[Lay::idx(c, base + r * Ns)] = a[r];
+        for (int r = 0; r < R; ++r) lds[lay.idx(c, base + r * Ns)] = a[r];
         Ns *= R;
       }
     }
@@ -300,7 +300,7 @@ __device__ __forceinline__ void comp_add(T hi, T lo, T y, T& ho, T& lo_out) {
   lo_out = e - (ho - s);
 }
 
-// Stage update at spectral element ubase + off (ubase uniform over the block,
+// Stage updates at spectral element ubase + off (ubase uniform over the block,
 // off a 32-bit lane offset; member-relative wavenumber indices ikx, iky, ikz)
 // given the nonlinear term N of the stage input.
 // RK4 ([FF] RK4substeps!/RK4update!): RHS_s = N_s + L x_s;
@@ -309,96 +309,143 @@ __device__ __forceinline__ void comp_add(T hi, T lo, T y, T& ho, T& lo_out) {
 // ETDRK4 ([FF] ETDRK4substeps!/ETDRK4update!):
 //   sol1 = E2 sol + zeta N1 ; sol2 = E2 sol + zeta N2 ; sol2 = E2 sol1 + zeta (2 N3 - N1);
 //   sol = E sol + alpha N1 + 2 beta (N2 + N3) + Gamma N4
+// Every update returns the value it stores as the next stage's input (for the
+// last stage: the new sol), so a fused kernel can go on transforming it.
+
+// RK4 epilogue split into an operand load (which callers may issue early) and
+// the update.
 template <typename T>
-__device__ __forceinline__ void stage_epilogue(const StageArgs<T>& a, long long ubase, uint32_t off, int member,
-                                               int ikx, int iky, int ikz, cpx<T> N) {
-  if (a.kind == EPI_RK4) {
-    const double Ld = linear_op(a.g, a.op[member], a.nkappah, ikx, iky, ikz);
-    const T L = (T)Ld;
-    const T dt = (T)a.dt;
-    cpx<T> x = a.stage == 0 ? (a.sol + ubase)[off] : (a.stg + ubase)[off];
-    cpx<T> rhs = {N.re + L * x.re, N.im + L * x.im};
-    if (a.stage == 0) {
-      const T h = (T)(a.dt / 2);
-      (a.stg + ubase)[off] = {x.re + h * rhs.re, x.im + h * rhs.im};
-      (a.acc + ubase)[off] = {rhs.re / (T)6, rhs.im / (T)6};
-    } else if (a.stage == 1 || a.stage == 2) {
-      const T h = a.stage == 1 ? (T)(a.dt / 2) : dt;
-      cpx<T> s = (a.sol + ubase)[off];
-      cpx<T> ac = (a.acc + ubase)[off];
-      (a.stg + ubase)[off] = {s.re + h * rhs.re, s.im + h * rhs.im};
-      (a.acc + ubase)[off] = {ac.re + rhs.re / (T)3, ac.im + rhs.im / (T)3};
-    } else {
-      cpx<T> s = (a.sol + ubase)[off];
-      cpx<T> ac = (a.acc + ubase)[off];
-      cpx<T> tot = {ac.re + rhs.re / (T)6, ac.im + rhs.im / (T)6};
-      if (a.sol_lo) {
-        // compensated accumulation: (hi, lo) += dt tot with TwoSum + renormalisation
-        const cpx<T> lo = (a.sol_lo + ubase)[off];
-        T hr, lr, hi_, li_;
-        comp_add(s.re, lo.re, dt * tot.re, hr, lr);
-        comp_add(s.im, lo.im, dt * tot.im, hi_, li_);
-        (a.sol + ubase)[off] = {hr, hi_};
-        (a.sol_lo + ubase)[off] = {lr, li_};
-      } else {
-        (a.sol + ubase)[off] = {s.re + dt * tot.re, s.im + dt * tot.im};
-      }
-    }
-  } else if (a.kind == EPI_ETDRK4) {
-    // coefficients are FP64 arrays; the update runs in FP64 and rounds once
-    const double nre = (double)N.re, nim = (double)N.im;
-    auto sol_at = [&](double& re, double& im) {
-      const cpx<T> s = (a.sol + ubase)[off];
-      re = (double)s.re;
-      im = (double)s.im;
-      if (a.sol_lo) {
-        const cpx<T> lo = (a.sol_lo + ubase)[off];
-        re += (double)lo.re;
-        im += (double)lo.im;
-      }
-    };
-    if (a.stage == 0 || a.stage == 1) {
-      double sr, si;
-      sol_at(sr, si);
-      const double e2 = (a.E2 + ubase)[off], z = (a.zeta + ubase)[off];
-      const cpx<T> v = {(T)(e2 * sr + z * nre), (T)(e2 * si + z * nim)};
-      if (a.stage == 0) {
-        (a.stg + ubase)[off] = v;
-        (a.n1 + ubase)[off] = N;
-      } else {
-        (a.stg2 + ubase)[off] = v;
-        (a.n2 + ubase)[off] = N;
-      }
-    } else if (a.stage == 2) {
-      const cpx<T> s1 = (a.stg + ubase)[off];
-      double sr, si;
-      sol_at(sr, si);
-      const cpx<T> N1 = (a.n1 + ubase)[off];
-      const cpx<T> N2 = (a.n2 + ubase)[off];
-      const double e2 = (a.E2 + ubase)[off], z = (a.zeta + ubase)[off];
-      const double e = (a.E + ubase)[off], al = (a.alpha + ubase)[off], be2 = 2.0 * (a.beta + ubase)[off];
-      (a.stg2 + ubase)[off] = {(T)(e2 * (double)s1.re + z * (2.0 * nre - (double)N1.re)),
-                     (T)(e2 * (double)s1.im + z * (2.0 * nim - (double)N1.im))};
-      const double ar = e * sr + al * (double)N1.re + be2 * ((double)N2.re + nre);
-      const double ai = e * si + al * (double)N1.im + be2 * ((double)N2.im + nim);
-      const cpx<T> hi = {(T)ar, (T)ai};
-      (a.acc + ubase)[off] = hi;
-      if (a.sol_lo) (a.n2 + ubase)[off] = {(T)(ar - (double)hi.re), (T)(ai - (double)hi.im)};  // low part
-    } else {
-      const cpx<T> ac = (a.acc + ubase)[off];
-      double ar = (double)ac.re, ai = (double)ac.im;
-      if (a.sol_lo) {
-        const cpx<T> al = (a.n2 + ubase)[off];
-        ar += (double)al.re;
-        ai += (double)al.im;
-      }
-      const double g = (a.gam + ubase)[off];
-      const double sr = ar + g * nre, si = ai + g * nim;
-      const cpx<T> hi = {(T)sr, (T)si};
-      (a.sol + ubase)[off] = hi;
-      if (a.sol_lo) (a.sol_lo + ubase)[off] = {(T)(sr - (double)hi.re), (T)(si - (double)hi.im)};
-    }
+struct Rk4Ops {
+  cpx<T> x, s, ac, lo;
+};
+
+template <typename T>
+__device__ __forceinline__ Rk4Ops<T> rk4_load(const StageArgs<T>& a, long long ubase, uint32_t off) {
+  // every field is defined on every path so that the value stays in registers
+  const cpx<T> z = {(T)0, (T)0};
+  Rk4Ops<T> o = {z, z, z, z};
+  if (a.stage == 0) {
+    o.x = (a.sol + ubase)[off];
+  } else {
+    o.x = (a.stg + ubase)[off];
+    o.s = (a.sol + ubase)[off];
+    o.ac = (a.acc + ubase)[off];
+    if (a.stage == 3 && a.sol_lo) o.lo = (a.sol_lo + ubase)[off];
   }
+  return o;
+}
+
+template <typename T>
+__device__ __forceinline__ cpx<T> rk4_finish(const StageArgs<T>& a, long long ubase, uint32_t off,
+                                             int member, int ikx, int iky, int ikz, cpx<T> N,
+                                             const Rk4Ops<T>& o) {
+  const T L = (T)linear_op(a.g, a.op[member], a.nkappah, ikx, iky, ikz);
+  const T dt = (T)a.dt;
+  const cpx<T> x = o.x;
+  const cpx<T> rhs = {N.re + L * x.re, N.im + L * x.im};
+  if (a.stage == 0) {
+    const T h = (T)(a.dt / 2);
+    const cpx<T> nx = {x.re + h * rhs.re, x.im + h * rhs.im};
+    (a.stg + ubase)[off] = nx;
+    (a.acc + ubase)[off] = {rhs.re / (T)6, rhs.im / (T)6};
+    return nx;
+  } else if (a.stage == 1 || a.stage == 2) {
+    const T h = a.stage == 1 ? (T)(a.dt / 2) : dt;
+    const cpx<T> nx = {o.s.re + h * rhs.re, o.s.im + h * rhs.im};
+    (a.stg + ubase)[off] = nx;
+    (a.acc + ubase)[off] = {o.ac.re + rhs.re / (T)3, o.ac.im + rhs.im / (T)3};
+    return nx;
+  } else {
+    const cpx<T> tot = {o.ac.re + rhs.re / (T)6, o.ac.im + rhs.im / (T)6};
+    if (a.sol_lo) {
+      // compensated accumulation: (hi, lo) += dt tot with TwoSum + renormalisation
+      T hr, lr, hi_, li_;
+      comp_add(o.s.re, o.lo.re, dt * tot.re, hr, lr);
+      comp_add(o.s.im, o.lo.im, dt * tot.im, hi_, li_);
+      (a.sol + ubase)[off] = {hr, hi_};
+      (a.sol_lo + ubase)[off] = {lr, li_};
+      return {hr, hi_};
+    }
+    const cpx<T> ns = {o.s.re + dt * tot.re, o.s.im + dt * tot.im};
+    (a.sol + ubase)[off] = ns;
+    return ns;
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ cpx<T> etdrk4_update(const StageArgs<T>& a, long long ubase,
+                                                uint32_t off, cpx<T> N) {
+  // coefficients are FP64 arrays; the update runs in FP64 and rounds once
+  const double nre = (double)N.re, nim = (double)N.im;
+  auto sol_at = [&](double& re, double& im) {
+    const cpx<T> s = (a.sol + ubase)[off];
+    re = (double)s.re;
+    im = (double)s.im;
+    if (a.sol_lo) {
+      const cpx<T> lo = (a.sol_lo + ubase)[off];
+      re += (double)lo.re;
+      im += (double)lo.im;
+    }
+  };
+  if (a.stage == 0 || a.stage == 1) {
+    double sr, si;
+    sol_at(sr, si);
+    const double e2 = (a.E2 + ubase)[off], z = (a.zeta + ubase)[off];
+    const cpx<T> v = {(T)(e2 * sr + z * nre), (T)(e2 * si + z * nim)};
+    if (a.stage == 0) {
+      (a.stg + ubase)[off] = v;
+      (a.n1 + ubase)[off] = N;
+    } else {
+      (a.stg2 + ubase)[off] = v;
+      (a.n2 + ubase)[off] = N;
+    }
+    return v;
+  } else if (a.stage == 2) {
+    const cpx<T> s1 = (a.stg + ubase)[off];
+    double sr, si;
+    sol_at(sr, si);
+    const cpx<T> N1 = (a.n1 + ubase)[off];
+    const cpx<T> N2 = (a.n2 + ubase)[off];
+    const double e2 = (a.E2 + ubase)[off], z = (a.zeta + ubase)[off];
+    const double e = (a.E + ubase)[off], al = (a.alpha + ubase)[off];
+    const double be2 = 2.0 * (a.beta + ubase)[off];
+    const cpx<T> v = {(T)(e2 * (double)s1.re + z * (2.0 * nre - (double)N1.re)),
+                      (T)(e2 * (double)s1.im + z * (2.0 * nim - (double)N1.im))};
+    (a.stg2 + ubase)[off] = v;
+    const double ar = e * sr + al * (double)N1.re + be2 * ((double)N2.re + nre);
+    const double ai = e * si + al * (double)N1.im + be2 * ((double)N2.im + nim);
+    const cpx<T> hi = {(T)ar, (T)ai};
+    (a.acc + ubase)[off] = hi;
+    if (a.sol_lo) (a.n2 + ubase)[off] = {(T)(ar - (double)hi.re), (T)(ai - (double)hi.im)};  // low part
+    return v;
+  } else {
+    const cpx<T> ac = (a.acc + ubase)[off];
+    double ar = (double)ac.re, ai = (double)ac.im;
+    if (a.sol_lo) {
+      const cpx<T> al = (a.n2 + ubase)[off];
+      ar += (double)al.re;
+      ai += (double)al.im;
+    }
+    const double g = (a.gam + ubase)[off];
+    const double sr = ar + g * nre, si = ai + g * nim;
+    const cpx<T> hi = {(T)sr, (T)si};
+    (a.sol + ubase)[off] = hi;
+    if (a.sol_lo) (a.sol_lo + ubase)[off] = {(T)(sr - (double)hi.re), (T)(si - (double)hi.im)};
+    return hi;
+  }
+}
+
+// Stage update at spectral element ubase + off with the stage's nonlinear term N;
+// returns the next stage's input at that element.
+template <typename T>
+__device__ __forceinline__ cpx<T> stage_epilogue(const StageArgs<T>& a, long long ubase,
+                                                 uint32_t off, int member, int ikx, int iky,
+                                                 int ikz, cpx<T> N) {
+  if (a.kind == EPI_RK4) {
+    const Rk4Ops<T> o = rk4_load(a, ubase, off);
+    return rk4_finish(a, ubase, off, member, ikx, iky, ikz, N, o);
+  }
+  return etdrk4_update(a, ubase, off, N);
 }
 
 }  // namespace ptf

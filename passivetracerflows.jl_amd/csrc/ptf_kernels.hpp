@@ -20,7 +20,8 @@ enum {
   COL_D1 = 1,      // out0 = IFFT(in0), out1 = IFFT(i k in0)                   (first inverse axis)
   COL_D2 = 2,      // out0 = IFFT(in0), out1 = IFFT(i k in0), out2 = IFFT(in1)  (3D y inverse)
   COL_STAGE = 3,   // FFT_fwd(in0) -> stage epilogue                            (last forward axis)
-  COL_FLOWSYN = 4  // psi(t) = psi0 exp(-i w t): out0 = IFFT_y(-i l psi), out1 = IFFT_y(i kr psi)
+  COL_FLOWSYN = 4, // psi(t) = psi0 exp(-i w t): out0 = IFFT_y(-i l psi), out1 = IFFT_y(i kr psi)
+  COL_STAGE_D1 = 5 // COL_STAGE, then COL_D1 on the next stage's input (stage boundary fusion)
 };
 
 template <typename T>
@@ -43,22 +44,31 @@ struct ColArgs {
   const double* omega;
   double t;
   double kscale_x;
+  // Brick layout (the last grid axis stored in bricks of 2^lb planes, innermost):
+  //   lb  > 0: the transformed line runs along the bricked axis; line element i
+  //            sits at column*2^lb + (i mod 2^lb) + (i >> lb)*stride
+  //   lbc > 0: the columns of this pass are (kx, i_last mod 2^lbc) pairs,
+  //            column = kx*2^lbc + (i_last mod 2^lbc), and blockIdx.y is i_last >> lbc
+  int lb;
+  int lbc;
 };
 
-// Occupancy request for a block of `threads` threads using `lds` bytes of LDS:
-// the waves per SIMD that the LDS budget (160 KiB per CU) admits, capped by
-// what the line registers of radix R in precision T need (so that the compiler
-// does not spill), in [1, 8].
-template <typename T>
-constexpr int occupancy_waves(int threads, int lds, int R, int live_arrays) {
+// Occupancy range for a block of `threads` threads using `lds` bytes of LDS:
+// the upper end is the waves per SIMD that the LDS budget (160 KiB per CU)
+// admits (at most 8); the lower end (at most 2) is what the register file must
+// always allow.  Within the range the compiler raises occupancy only as far as
+// it can without spilling.
+constexpr int lds_waves(int threads, int lds) {
   int blocks = lds > 0 ? 163840 / lds : 16;
   if (blocks < 1) blocks = 1;
   if (blocks > 16) blocks = 16;
   int w = blocks * ((threads + 63) / 64) / 4;
-  const int regs = live_arrays * R * (int)(2 * sizeof(T) / 4) + 48;
-  const int wr = 512 / regs;
-  if (w > wr) w = wr;
   return w < 1 ? 1 : (w > 8 ? 8 : w);
+}
+constexpr int min_waves(int threads, int lds, int occ = 0) {
+  const int w = lds_waves(threads, lds);
+  const int want = occ > 0 ? occ : 2;   // occ > 0: a tighter register budget, spills allowed
+  return w < want ? w : want;
 }
 
 // Workgroups are dispatched round-robin over the 8 XCDs of the device, each
@@ -81,18 +91,26 @@ __device__ __forceinline__ void xcd_remap(int& bx, int& by, int& bz) {
 template <typename T, int N, int R, int TX>
 constexpr int col_lds_elems() { return (N > R) ? N * TX : 1; }
 
-template <typename T, int N, int R, int TX, int MODE, int DIR>
+template <typename T, int N, int R, int TX, int MODE, int DIR, int OCC = 0>
 __global__ __attribute__((amdgpu_flat_work_group_size(1, TX * (N / R)),
-                          amdgpu_waves_per_eu(occupancy_waves<T>(TX * (N / R),
-                                              (int)(col_lds_elems<T, N, R, TX>() * sizeof(cpx<T>)),
-                                              R, 3))))
+                          amdgpu_waves_per_eu(
+                              min_waves(TX * (N / R), (int)(col_lds_elems<T, N, R, TX>() * sizeof(cpx<T>)), OCC),
+                              lds_waves(TX * (N / R), (int)(col_lds_elems<T, N, R, TX>() * sizeof(cpx<T>))))))
 void col_kernel(ColArgs<T> a) {
   constexpr int P = N / R;
   using Lay = LayInterleaved<TX>;
+  const Lay lay{};
   __shared__ cpx<T> lds[col_lds_elems<T, N, R, TX>()];
   const int t = threadIdx.x;
-  const int c = t % TX;
-  const int j = t / TX;
+  // lane t = rb + 2^lb (c + TX jh): the brick's planes vary fastest, then the
+  // columns, so a wavefront reads 2^lb * TX contiguous values; the line
+  // element of register q is i = j + q P with j = rb + 2^lb jh (P % 2^lb == 0)
+  const int lb = a.lb;
+  const int rb = t & ((1 << lb) - 1);
+  const int rest = t >> lb;
+  const int c = rest % TX;
+  const int jh = rest / TX;
+  const int j = rb + (jh << lb);
   int bx, by, member;
   xcd_remap(bx, by, member);
   const int kx = bx * TX + c;
@@ -101,7 +119,12 @@ void col_kernel(ColArgs<T> a) {
   // block-uniform base (SGPRs) + 32-bit lane offsets
   const long long ubase = (long long)member * a.mstride + (long long)by * a.ostride;
   const uint32_t s = (uint32_t)a.stride;
-  auto off = [&](int q) -> uint32_t { return (uint32_t)kxc + (uint32_t)(j + q * P) * s; };
+  auto off = [&](int q) -> uint32_t {
+    return ((uint32_t)kxc << lb) + (uint32_t)rb + (uint32_t)(jh + q * (P >> lb)) * s;
+  };
+  // wavenumber indices of this lane's column (3D y pass over folded columns)
+  const int ikx_col = kx >> a.lbc;
+  const int iklast_col = (by << a.lbc) + (kx & ((1 << a.lbc) - 1));
 
   cpx<T> v[R];
   if constexpr (MODE == COL_FLOWSYN) {
@@ -119,14 +142,14 @@ void col_kernel(ColArgs<T> a) {
         sincos(ph, &sn, &cs);
         const cpx<double> p = {p0.re * cs - p0.im * sn, p0.re * sn + p0.im * cs};
         if (pass == 0) {
-          const double kr = kx * a.kscale_x;
+          const double kr = ikx_col * a.kscale_x;
           v[q] = {(T)(-(kr * p.im)), (T)(kr * p.re)};          // i kr p
         } else {
           const double l = fftfreq_index(iy, N) * a.kscale;
           v[q] = {(T)(l * p.im), (T)(-(l * p.re))};             // -i l p
         }
       }
-      line_fft<T, N, R, +1, Lay>(v, lds, c, j, a.tw, a.twstride);
+      line_fft<T, N, R, +1, Lay>(v, lds, lay, c, j, a.tw, a.twstride);
       cpx<T>* o = (pass == 0 ? a.out1 : a.out0) + ubase;
       if (valid) {
 #pragma unroll
@@ -140,21 +163,34 @@ void col_kernel(ColArgs<T> a) {
     for (int q = 0; q < R; ++q) v[q] = in0[off(q)];
 
     if constexpr (MODE == COL_PLAIN) {
-      line_fft<T, N, R, DIR, Lay>(v, lds, c, j, a.tw, a.twstride);
+      line_fft<T, N, R, DIR, Lay>(v, lds, lay, c, j, a.tw, a.twstride);
       cpx<T>* o = a.out0 + ubase;
       if (valid) {
 #pragma unroll
         for (int q = 0; q < R; ++q) o[off(q)] = v[q];
       }
     } else if constexpr (MODE == COL_D1 || MODE == COL_D2) {
-      // derivative first: out0 may alias in0 (in place), so in0 is re-read
-      // for the plain transform only after out1 has been written
+      // The derivative is transformed and stored first, because out0 may alias in0
+      // (in place).  When registers allow, the plain copy (and in1) stay in
+      // registers from the first load instead of being re-read after a store.
+      constexpr bool kKeep = sizeof(T) == 4 || R <= 8;
+      cpx<T> u[kKeep ? R : 1];
+      cpx<T> p2[(MODE == COL_D2 && kKeep) ? R : 1];
+      if constexpr (kKeep) {
+#pragma unroll
+        for (int q = 0; q < R; ++q) u[q] = v[q];
+        if constexpr (MODE == COL_D2) {
+          const cpx<T>* in1 = a.in1 + ubase;
+#pragma unroll
+          for (int q = 0; q < R; ++q) p2[q] = in1[off(q)];
+        }
+      }
 #pragma unroll
       for (int q = 0; q < R; ++q) {
         const T k = (T)(fftfreq_index(j + q * P, N) * a.kscale);
         v[q] = times_ik(v[q], k);
       }
-      line_fft<T, N, R, +1, Lay>(v, lds, c, j, a.tw, a.twstride);
+      line_fft<T, N, R, +1, Lay>(v, lds, lay, c, j, a.tw, a.twstride);
       {
         cpx<T>* o = a.out1 + ubase;
         if (valid) {
@@ -163,8 +199,11 @@ void col_kernel(ColArgs<T> a) {
         }
       }
 #pragma unroll
-      for (int q = 0; q < R; ++q) v[q] = in0[off(q)];
-      line_fft<T, N, R, +1, Lay>(v, lds, c, j, a.tw, a.twstride);
+      for (int q = 0; q < R; ++q) {
+        if constexpr (kKeep) v[q] = u[q];
+        else v[q] = in0[off(q)];
+      }
+      line_fft<T, N, R, +1, Lay>(v, lds, lay, c, j, a.tw, a.twstride);
       {
         cpx<T>* o = a.out0 + ubase;
         if (valid) {
@@ -175,28 +214,89 @@ void col_kernel(ColArgs<T> a) {
       if constexpr (MODE == COL_D2) {
         const cpx<T>* in1 = a.in1 + ubase;
 #pragma unroll
-        for (int q = 0; q < R; ++q) v[q] = in1[off(q)];
-        line_fft<T, N, R, +1, Lay>(v, lds, c, j, a.tw, a.twstride);
+        for (int q = 0; q < R; ++q) {
+          if constexpr (kKeep) v[q] = p2[q];
+          else v[q] = in1[off(q)];
+        }
+        line_fft<T, N, R, +1, Lay>(v, lds, lay, c, j, a.tw, a.twstride);
         cpx<T>* o = a.out2 + ubase;
         if (valid) {
 #pragma unroll
           for (int q = 0; q < R; ++q) o[off(q)] = v[q];
         }
       }
-    } else if constexpr (MODE == COL_STAGE) {
-      line_fft<T, N, R, -1, Lay>(v, lds, c, j, a.tw, a.twstride);
-      if (valid) {
-        const int o = by;
+    } else if constexpr (MODE == COL_STAGE || MODE == COL_STAGE_D1) {
+      // FP32 and radix 4: the RK4 operands of the epilogue are loaded before the
+      // transform so that their latency overlaps it (FP64 radix 8/16 would spill)
+      constexpr bool kPre = sizeof(T) == 4 || R <= 4;
+      Rk4Ops<T> ops[kPre ? R : 1];
+      const bool pre = kPre && a.st.kind == EPI_RK4;
+      if constexpr (kPre) {
+        if (pre) {
 #pragma unroll
-        for (int q = 0; q < R; ++q) {
-          const int ia = j + q * P;
+          for (int q = 0; q < R; ++q) ops[q] = rk4_load(a.st, ubase, off(q));
+        }
+      }
+      line_fft<T, N, R, -1, Lay>(v, lds, lay, c, j, a.tw, a.twstride);
+      // invalid (tail) lanes compute on a clamped column and store nothing
+#pragma unroll
+      for (int q = 0; q < R; ++q) {
+        const int ia = j + q * P;
+        const int iky = a.axis == 2 ? by : ia;
+        const int ikz = a.axis == 2 ? ia : iklast_col;
+        if constexpr (MODE == COL_STAGE) {
+          if (!valid) continue;
           if (a.st.kind == EPI_STORE) {
             (a.out0 + ubase)[off(q)] = v[q];
-          } else if (a.axis == 2) {
-            stage_epilogue(a.st, ubase, off(q), member, kx, o, ia, v[q]);
-          } else {
-            stage_epilogue(a.st, ubase, off(q), member, kx, ia, o, v[q]);
+            continue;
           }
+        }
+        cpx<T> nx;
+        if (!valid) {
+          nx = v[q];
+        } else if (pre) {
+          if constexpr (kPre) nx = rk4_finish(a.st, ubase, off(q), member, ikx_col, iky, ikz, v[q], ops[q]);
+        } else {
+          nx = stage_epilogue(a.st, ubase, off(q), member, ikx_col, iky, ikz, v[q]);
+        }
+        v[q] = nx;
+      }
+      if constexpr (MODE == COL_STAGE_D1) {
+        // v holds the next stage's input along the line: transform it as COL_D1
+        constexpr bool kKeep = sizeof(T) == 4 || R <= 8;
+        cpx<T> u[kKeep ? R : 1];
+        if constexpr (kKeep) {
+#pragma unroll
+          for (int q = 0; q < R; ++q) u[q] = v[q];
+        }
+        cpx<T> w[kKeep ? 1 : R];
+        if constexpr (!kKeep) {
+#pragma unroll
+          for (int q = 0; q < R; ++q) w[q] = v[q];
+        }
+#pragma unroll
+        for (int q = 0; q < R; ++q) {
+          const T k = (T)(fftfreq_index(j + q * P, N) * a.kscale);
+          v[q] = times_ik(v[q], k);
+        }
+        line_fft<T, N, R, +1, Lay>(v, lds, lay, c, j, a.tw, a.twstride);
+        {
+          cpx<T>* op = a.out1 + ubase;
+          if (valid) {
+#pragma unroll
+            for (int q = 0; q < R; ++q) op[off(q)] = v[q];
+          }
+        }
+#pragma unroll
+        for (int q = 0; q < R; ++q) {
+          if constexpr (kKeep) v[q] = u[q];
+          else v[q] = w[q];
+        }
+        line_fft<T, N, R, +1, Lay>(v, lds, lay, c, j, a.tw, a.twstride);
+        cpx<T>* op = a.out0 + ubase;
+        if (valid) {
+#pragma unroll
+          for (int q = 0; q < R; ++q) op[off(q)] = v[q];
         }
       }
     }
@@ -228,6 +328,11 @@ struct RowArgs {
   double kscale_x;
   int ny;                  // for the epilogue's (iky, ikz) of a row
   StageArgs<T> st;         // ROW_ADV epilogue (1D problems)
+  // Brick layout of the last axis (2^lb planes innermost): rows are numbered
+  // (last_hi, middle, last_lo) with last_lo fastest; spectral element k of row r
+  // sits at (r >> lb) * nxh * 2^lb + k * 2^lb + (r mod 2^lb); physical values are
+  // stored as x-pairs, pair m of row r at ((r >> lb) * nx/2 + m) * 2^lb + (r mod 2^lb).
+  int lb;
 };
 
 // C2R pre-processing of one element pair: Z[k] from X[k], X[M-k] (k in [0, M)),
@@ -259,67 +364,93 @@ __device__ __forceinline__ cpx<T> r2c_post(cpx<T> Zk, cpx<T> Zm, cpx<T> w) {
   return {(T)0.5 * (e.re + od.im), (T)0.5 * (e.im - od.re)};
 }
 
-// Load spectral row entries X[k], k = j + q P, into x[] and the partners
-// X[M - k] into xm[] through LDS (the row's last entry X[M] is written by
-// lane j = 0).  Starts and ends with a barrier: LDS is free afterwards.
+// Partners X[M - k] of the row entries x[q] = X[j + q P] through LDS (X[M] is
+// passed in `last` by lane j = 0).  Starts and ends with a barrier: LDS is
+// free afterwards.
 template <typename T, int M, int R, class Lay>
-__device__ __forceinline__ void load_row_pairs(const cpx<T>* __restrict__ row, cpx<T> (&x)[R],
-                                               cpx<T> (&xm)[R], cpx<T>* lds, int c, int j) {
+__device__ __forceinline__ void stage_row_pairs(const cpx<T> (&x)[R], cpx<T> last, cpx<T> (&xm)[R],
+                                                cpx<T>* lds, const Lay& lay, int c, int j) {
   constexpr int P = M / R;
-#pragma unroll
-  for (int q = 0; q < R; ++q) x[q] = row[j + q * P];
-  cpx<T> last;
-  if (j == 0) last = row[M];
   __syncthreads();
 #pragma unroll
-  for (int q = 0; q < R; ++q) lds[Lay::idx(c, j + q * P)] = x[q];
-  if (j == 0) lds[Lay::idx(c, M)] = last;
+  for (int q = 0; q < R; ++q) lds[lay.idx(c, j + q * P)] = x[q];
+  if (j == 0) lds[lay.idx(c, M)] = last;
   __syncthreads();
 #pragma unroll
-  for (int q = 0; q < R; ++q) xm[q] = lds[Lay::idx(c, M - (j + q * P))];
+  for (int q = 0; q < R; ++q) xm[q] = lds[lay.idx(c, M - (j + q * P))];
 }
 
-template <typename T, int NX, int R, int NL, int MODE, int NIN>
+// Row element k is row[k << lb] (brick layout; lb = 0 for contiguous rows).
+template <typename T, int M, int R>
+__device__ __forceinline__ void load_row(const cpx<T>* __restrict__ row, cpx<T> (&x)[R],
+                                         cpx<T>& last, int j, int lb) {
+  constexpr int P = M / R;
+#pragma unroll
+  for (int q = 0; q < R; ++q) x[q] = row[(uint32_t)(j + q * P) << lb];
+  if (j == 0) last = row[(uint32_t)M << lb];
+}
+
+template <typename T, int M, int R, class Lay>
+__device__ __forceinline__ void load_row_pairs(const cpx<T>* __restrict__ row, cpx<T> (&x)[R],
+                                               cpx<T> (&xm)[R], cpx<T>* lds, const Lay& lay, int c,
+                                               int j, int lb) {
+  cpx<T> last = {(T)0, (T)0};
+  load_row<T, M, R>(row, x, last, j, lb);
+  stage_row_pairs<T, M, R, Lay>(x, last, xm, lds, lay, c, j);
+}
+
+template <typename T, int NX, int R, int NL, int MODE, int NIN, int OCC = 0>
 __global__ __attribute__((amdgpu_flat_work_group_size(1, NL * ((NX / 2) / R)),
-                          amdgpu_waves_per_eu(occupancy_waves<T>(NL * ((NX / 2) / R),
-                                              (int)(NL * (NX / 2 + 1) * sizeof(cpx<T>)), R, 5))))
+                          amdgpu_waves_per_eu(
+                              min_waves(NL * ((NX / 2) / R), (int)(NL * (NX / 2 + 1) * sizeof(cpx<T>)), OCC),
+                              lds_waves(NL * ((NX / 2) / R), (int)(NL * (NX / 2 + 1) * sizeof(cpx<T>))))))
 void row_kernel(RowArgs<T> a) {
   constexpr int M = NX / 2;
   constexpr int NXH = M + 1;
   constexpr int P = M / R;
-  constexpr int PAD = 1;   // row stride M + 1 also holds X[M] while staging a row
-  using Lay = LayRows<M, PAD>;
+  // LDS rows hold M + 1 entries (X[M] while staging a row).  Contiguous rows
+  // (lanes walk a row) are stored row-major; bricked rows (lanes walk the rows
+  // of a brick first) element-major, so a wavefront's accesses stay contiguous.
+  constexpr int PAD = 1;
+  using Lay = LayStrided;
+  const Lay lay = a.lb ? Lay{1, NL} : Lay{M + PAD, 1};
   __shared__ cpx<T> lds[NL * (M + PAD)];
   const int t = threadIdx.x;
-  const int j = t % P;
-  const int c = t / P;
+  // lane t = rb + 2^lb (j + P g): the rows of a brick vary fastest (NL % 2^lb == 0)
+  const int lb = a.lb;
+  const int rb = t & ((1 << lb) - 1);
+  const int rest = t >> lb;
+  const int j = rest % P;
+  const int g = rest / P;
+  const int c = (g << lb) + rb;                                  // row within the block
   const long long row = (long long)blockIdx.x * NL + c;
   const bool valid = row < a.nrows;
-  const int cc = valid ? c : (int)(a.nrows - 1 - (long long)blockIdx.x * NL);  // clamped row
+  const int gmax = (int)((a.nrows - (long long)blockIdx.x * NL) >> lb) - 1;
+  const int gc = valid ? g : (g < gmax ? g : gmax);              // clamped brick row
   const int member = blockIdx.y;
   // block-uniform bases + 32-bit lane offsets
   const long long sub = (long long)member * a.smstride + (long long)blockIdx.x * NL * NXH;
   const long long pub = (long long)member * a.pmstride + (long long)blockIdx.x * NL * NX;
-  const uint32_t so = (uint32_t)cc * NXH;   // spectral row offset
-  const uint32_t po = (uint32_t)cc * NX;    // physical row offset
-  const uint32_t so_st = (uint32_t)c * NXH;
-  const uint32_t po_st = (uint32_t)c * NX;
+  const uint32_t so = ((uint32_t)gc * NXH << lb) + rb;           // spectral element 0 (loads)
+  const uint32_t so_st = ((uint32_t)g * NXH << lb) + rb;         // (stores)
+  const uint32_t po = ((uint32_t)gc * M << lb) + rb;             // physical pair 0 (loads)
+  const uint32_t po_st = ((uint32_t)g * M << lb) + rb;           // (stores)
   const cpx<T> zero = {(T)0, (T)0};
 
   if constexpr (MODE == ROW_C2R) {
     cpx<T> z[R], zm[R];
     const T s = (T)a.scale;
-    load_row_pairs<T, M, R, Lay>(a.in[0] + sub + so, z, zm, lds, c, j);
+    load_row_pairs<T, M, R, Lay>(a.in[0] + sub + so, z, zm, lds, lay, c, j, lb);
 #pragma unroll
     for (int q = 0; q < R; ++q) {
       const int k = j + q * P;
       z[q] = c2r_pre(z[q], zm[q], a.tw[k * a.twstride], s, k == 0);
     }
-    line_fft<T, M, R, +1, Lay>(z, lds, c, j, a.tw, 2 * a.twstride);
+    line_fft<T, M, R, +1, Lay>(z, lds, lay, c, j, a.tw, 2 * a.twstride);
     if (valid) {
       cpx<T>* out = reinterpret_cast<cpx<T>*>(a.rout + pub);
 #pragma unroll
-      for (int q = 0; q < R; ++q) out[(po_st >> 1) + j + q * P] = z[q];
+      for (int q = 0; q < R; ++q) out[po_st + ((uint32_t)(j + q * P) << lb)] = z[q];
     }
     return;
   } else {
@@ -327,14 +458,37 @@ void row_kernel(RowArgs<T> a) {
     if constexpr (MODE == ROW_R2C) {
       const cpx<T>* rin = reinterpret_cast<const cpx<T>*>(a.rin + pub);
 #pragma unroll
-      for (int q = 0; q < R; ++q) f[q] = rin[(po >> 1) + j + q * P];
+      for (int q = 0; q < R; ++q) f[q] = rin[po + ((uint32_t)(j + q * P) << lb)];
     } else {
       // ROW_ADV: f = ((-u cx) - v cy) - w cz, cx = C2R(i kr Y0), cy = C2R(Y1), cz = C2R(Y2)
       const T s = (T)a.scale;
+      // select, not a.in[d]: a runtime index into the by-value kernel arguments
+      // would copy them to scratch
+      auto in_at = [&](int d) { return d == 0 ? a.in[0] : (d == 1 ? a.in[1] : a.in[2]); };
+      auto flow_at = [&](int d) { return d == 0 ? a.flow[0] : (d == 1 ? a.flow[1] : a.flow[2]); };
+      // FP32 and radix 4: the next input row and the flow values are loaded before
+      // the current row is transformed (FP64 radix 8/16 would spill)
+      constexpr bool kPre = sizeof(T) == 4 || R <= 4;
+      cpx<T> nxt[kPre ? R : 1];
+      cpx<T> nlast = zero;
+      if constexpr (kPre) load_row<T, M, R>(a.in[0] + sub + so, nxt, nlast, j, lb);
 #pragma unroll 1
       for (int d = 0; d < NIN; ++d) {
         cpx<T> z[R], zm[R];
-        load_row_pairs<T, M, R, Lay>(a.in[d] + sub + so, z, zm, lds, c, j);
+        if constexpr (kPre) {
+#pragma unroll
+          for (int q = 0; q < R; ++q) z[q] = nxt[q];
+          stage_row_pairs<T, M, R, Lay>(z, nlast, zm, lds, lay, c, j);
+          if (d + 1 < NIN) load_row<T, M, R>(in_at(d + 1) + sub + so, nxt, nlast, j, lb);
+        } else {
+          load_row_pairs<T, M, R, Lay>(in_at(d) + sub + so, z, zm, lds, lay, c, j, lb);
+        }
+        const cpx<T>* fl = reinterpret_cast<const cpx<T>*>(flow_at(d) + pub);
+        cpx<T> fv[kPre ? R : 1];
+        if constexpr (kPre) {
+#pragma unroll
+          for (int q = 0; q < R; ++q) fv[q] = fl[po + ((uint32_t)(j + q * P) << lb)];
+        }
 #pragma unroll
         for (int q = 0; q < R; ++q) {
           const int k = j + q * P;
@@ -346,11 +500,12 @@ void row_kernel(RowArgs<T> a) {
           }
           z[q] = c2r_pre(Xk, Xm, a.tw[k * a.twstride], s, k == 0);
         }
-        line_fft<T, M, R, +1, Lay>(z, lds, c, j, a.tw, 2 * a.twstride);
-        const cpx<T>* fl = reinterpret_cast<const cpx<T>*>(a.flow[d] + pub);
+        line_fft<T, M, R, +1, Lay>(z, lds, lay, c, j, a.tw, 2 * a.twstride);
 #pragma unroll
         for (int q = 0; q < R; ++q) {
-          const cpx<T> uu = fl[(po >> 1) + j + q * P];
+          cpx<T> uu;
+          if constexpr (kPre) uu = fv[q];
+          else uu = fl[po + ((uint32_t)(j + q * P) << lb)];
           if (d == 0) {
             f[q] = {-uu.re * z[q].re, -uu.im * z[q].im};
           } else {
@@ -360,11 +515,11 @@ void row_kernel(RowArgs<T> a) {
       }
     }
     // forward half-length FFT of f[2m] + i f[2m+1]
-    line_fft<T, M, R, -1, Lay>(f, lds, c, j, a.tw, 2 * a.twstride);
+    line_fft<T, M, R, -1, Lay>(f, lds, lay, c, j, a.tw, 2 * a.twstride);
     // exchange Z[M-k] through LDS
     __syncthreads();
 #pragma unroll
-    for (int q = 0; q < R; ++q) lds[Lay::idx(c, j + q * P)] = f[q];
+    for (int q = 0; q < R; ++q) lds[lay.idx(c, j + q * P)] = f[q];
     __syncthreads();
     if (valid) {
       const int iky = (int)(row % a.ny), ikz = (int)(row / a.ny);
@@ -375,7 +530,7 @@ void row_kernel(RowArgs<T> a) {
         if (k == 0) {
           X = {f[q].re + f[q].im, (T)0};
         } else {
-          const cpx<T> Zm = lds[Lay::idx(c, M - k)];
+          const cpx<T> Zm = lds[lay.idx(c, M - k)];
           X = r2c_post(f[q], Zm, a.tw[k * a.twstride]);
         }
         // only 1D problems finish a stage in the x kernel (x is their last forward axis)
@@ -383,15 +538,15 @@ void row_kernel(RowArgs<T> a) {
         bool epi = false;
         if constexpr (kEpi) epi = a.st.kind != EPI_STORE;
         if constexpr (kEpi) {
-          if (epi) stage_epilogue(a.st, sub, so_st + k, member, k, iky, ikz, X);
+          if (epi) (void)stage_epilogue(a.st, sub, so_st + ((uint32_t)k << lb), member, k, iky, ikz, X);
         }
-        if (!epi) (a.out + sub)[so_st + k] = X;
+        if (!epi) (a.out + sub)[so_st + ((uint32_t)k << lb)] = X;
         if (k == 0) {
           const cpx<T> XM = {f[q].re - f[q].im, (T)0};
           if constexpr (kEpi) {
-            if (epi) stage_epilogue(a.st, sub, so_st + M, member, M, iky, ikz, XM);
+            if (epi) (void)stage_epilogue(a.st, sub, so_st + ((uint32_t)M << lb), member, M, iky, ikz, XM);
           }
-          if (!epi) (a.out + sub)[so_st + M] = XM;
+          if (!epi) (a.out + sub)[so_st + ((uint32_t)M << lb)] = XM;
         }
       }
     }

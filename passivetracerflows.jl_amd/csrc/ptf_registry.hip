@@ -2,7 +2,8 @@
 // pair, with -DPTF_T=<double|float> -DPTF_TNAME=<f64|f32> -DPTF_KIND=<k>, and
 // exports  const void* ptf_reg_<TNAME>_<KIND>(int N, int R, int W)  returning
 // the kernel for line length N (column kinds: axis length; row kinds: nx),
-// radix R and width W (columns per block TX, or rows per block NL), or NULL
+// radix R and width W (columns per block TX, or rows per block NL; plus
+// 1000 * occ for the variants built with a 3-waves/SIMD budget), or NULL
 // when that variant does not exist.  Only variants that satisfy the launch
 // and LDS limits are instantiated.
 #include "ptf_kernels.hpp"
@@ -17,7 +18,7 @@ namespace {
 
 // kinds
 constexpr int K_COL_FWD = 0, K_COL_INV = 1, K_COL_D1 = 2, K_COL_D2 = 3, K_COL_STAGE = 4,
-              K_COL_FLOWSYN = 5;
+              K_COL_FLOWSYN = 5, K_COL_STAGE_D1 = 6;
 constexpr int K_ROW_C2R = 10, K_ROW_R2C = 11, K_ROW_ADV1 = 12, K_ROW_ADV2 = 13, K_ROW_ADV3 = 14;
 
 constexpr int col_mode(int k) {
@@ -25,60 +26,77 @@ constexpr int col_mode(int k) {
          : k == K_COL_D1                  ? COL_D1
          : k == K_COL_D2                  ? COL_D2
          : k == K_COL_STAGE               ? COL_STAGE
+         : k == K_COL_STAGE_D1            ? COL_STAGE_D1
                                           : COL_FLOWSYN;
 }
-constexpr int col_dir(int k) { return (k == K_COL_FWD || k == K_COL_STAGE) ? -1 : +1; }
+constexpr int col_dir(int k) {
+  return (k == K_COL_FWD || k == K_COL_STAGE || k == K_COL_STAGE_D1) ? -1 : +1;
+}
 
 constexpr int row_mode(int k) { return k == K_ROW_C2R ? ROW_C2R : k == K_ROW_R2C ? ROW_R2C : ROW_ADV; }
 constexpr int row_nin(int k) { return k == K_ROW_ADV2 ? 2 : k == K_ROW_ADV3 ? 3 : 1; }
 
-template <typename T, int N, int R, int TX, int KIND>
+// kinds that are also built with a 3-waves/SIMD register budget (OCC = 3)
+constexpr bool has_occ(int k) { return k == 3 || k == 6 || k == 13 || k == 14; }
+
+template <typename T, int N, int R, int TX, int KIND, int OCC>
 const void* colp() {
   constexpr int P = N / R;
-  constexpr bool ok = R <= N && (R >= 8 || N <= 16) && TX * P >= 16 && TX * P <= 1024 &&
+  constexpr bool ok = R <= N && (R >= 4) && TX * P >= 16 && TX * P <= 1024 &&
                       (long long)N * TX * (long long)sizeof(cpx<T>) <= 163840;
   if constexpr (ok) {
-    return reinterpret_cast<const void*>(&col_kernel<T, N, R, TX, col_mode(KIND), col_dir(KIND)>);
+    return reinterpret_cast<const void*>(&col_kernel<T, N, R, TX, col_mode(KIND), col_dir(KIND), OCC>);
   } else {
     return nullptr;
   }
 }
 
-template <typename T, int NX, int R, int NL, int KIND>
+template <typename T, int NX, int R, int NL, int KIND, int OCC>
 const void* rowp() {
   constexpr int M = NX / 2;
   constexpr int P = M / R;
-  constexpr bool ok = R <= M && (R >= 4) && NL * P >= 16 && NL * P <= 512 &&
-                      (long long)NL * (M + 1) * (long long)sizeof(cpx<T>) <= 65536;
+  constexpr bool ok = R <= M && (R >= 4) && NL * P >= 16 && NL * P <= 1024 &&
+                      (long long)NL * (M + 1) * (long long)sizeof(cpx<T>) <= 163840;
   if constexpr (ok) {
-    return reinterpret_cast<const void*>(&row_kernel<T, NX, R, NL, row_mode(KIND), row_nin(KIND)>);
+    return reinterpret_cast<const void*>(&row_kernel<T, NX, R, NL, row_mode(KIND), row_nin(KIND), OCC>);
   } else {
     return nullptr;
   }
 }
 
-template <typename T, int N, int R, int KIND>
-const void* by_w(int W) {
+template <typename T, int N, int R, int KIND, int OCC>
+const void* by_w_occ(int W) {
   if constexpr (KIND < 10) {
     switch (W) {
-      case 2: return colp<T, N, R, 2, KIND>();
-      case 4: return colp<T, N, R, 4, KIND>();
-      case 8: return colp<T, N, R, 8, KIND>();
-      case 16: return colp<T, N, R, 16, KIND>();
-      case 32: return colp<T, N, R, 32, KIND>();
-      case 64: return colp<T, N, R, 64, KIND>();
+      case 2: return colp<T, N, R, 2, KIND, OCC>();
+      case 4: return colp<T, N, R, 4, KIND, OCC>();
+      case 8: return colp<T, N, R, 8, KIND, OCC>();
+      case 16: return colp<T, N, R, 16, KIND, OCC>();
+      case 32: return colp<T, N, R, 32, KIND, OCC>();
+      case 64: return colp<T, N, R, 64, KIND, OCC>();
       default: return nullptr;
     }
   } else {
     switch (W) {
-      case 1: return rowp<T, N, R, 1, KIND>();
-      case 2: return rowp<T, N, R, 2, KIND>();
-      case 4: return rowp<T, N, R, 4, KIND>();
-      case 8: return rowp<T, N, R, 8, KIND>();
-      case 16: return rowp<T, N, R, 16, KIND>();
+      case 1: return rowp<T, N, R, 1, KIND, OCC>();
+      case 2: return rowp<T, N, R, 2, KIND, OCC>();
+      case 4: return rowp<T, N, R, 4, KIND, OCC>();
+      case 8: return rowp<T, N, R, 8, KIND, OCC>();
+      case 16: return rowp<T, N, R, 16, KIND, OCC>();
       default: return nullptr;
     }
   }
+}
+
+// W encodes the width and the occupancy request: W = width + 1000 * occ
+template <typename T, int N, int R, int KIND>
+const void* by_w(int W) {
+  const int occ = W / 1000, w = W % 1000;
+  if (occ == 0) return by_w_occ<T, N, R, KIND, 0>(w);
+  if constexpr (has_occ(KIND)) {
+    if (occ == 3) return by_w_occ<T, N, R, KIND, 3>(w);
+  }
+  return nullptr;
 }
 
 template <typename T, int N, int KIND>

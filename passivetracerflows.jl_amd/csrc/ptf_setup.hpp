@@ -84,16 +84,17 @@ __global__ void copy_kernel(T* __restrict__ out, const T* __restrict__ in, long 
 }
 
 // Neumaier-compensated FP64 partial sums of w_k |sol_k|^2 (Hermitian weights
-// 1 at kx = 0 and nx/2, 2 elsewhere), one (sum, compensation) pair per block.
+// 1 at kx = 0 and nx/2, 2 elsewhere), one (sum, compensation) pair per block;
+// the field may be in the brick layout (kx of element i is (i >> lb) mod nxh).
 template <typename T>
 __global__ void variance_partial_kernel(const cpx<T>* __restrict__ sol, long long n, int nxh,
-                                        double* __restrict__ partial) {
+                                        int lb, double* __restrict__ partial) {
   __shared__ double ssum[256];
   __shared__ double scomp[256];
   double sum = 0.0, comp = 0.0;
   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (long long)gridDim.x * blockDim.x) {
-    const int kx = (int)(i % nxh);
+    const int kx = (int)((i >> lb) % nxh);   // brick layout: 2^lb planes innermost
     const double w = (kx == 0 || kx == nxh - 1) ? 1.0 : 2.0;
     const cpx<T> v = sol[i];
     const double x = w * ((double)v.re * (double)v.re + (double)v.im * (double)v.im);
