@@ -32,6 +32,10 @@ for p in (ROOT, os.path.join(ROOT, "passivetracerflows.jl_amd")):
 
 METRIC = "grid-point-steps/s, FP64 plain vs FP32 batched at relL2≤1e-5; %HBM roofline"
 HBM_PEAK_GBS = 8000.0
+# fixed yardstick for the whole step: every FFT axis pass reads and writes its fields once,
+# flows read once per stage, RK4 state 4/6/6/5 S-fields per stage: 77 S + 12 R per step
+# (S = 16 B x 129/128 per point, R = 8 B per point at 256^3 FP64)
+PASS_MODEL_B_PER_GPS = 716.8
 
 
 def kernel_bytes(name: str, nS: int, nR: int, cb: int, rb: int, stage: int):
@@ -54,10 +58,10 @@ def kernel_bytes(name: str, nS: int, nR: int, cb: int, rb: int, stage: int):
         return 2 * S + R + (2 * S if stage == 0 else 3 * S)
     if fam == "y_forward":
         return 2 * S
-    state = {0: 3 * S, 1: 5 * S, 2: 5 * S, 3: 4 * S}[stage]
+    # RK4 state traffic of the FP64 form: stage 0 reads sol, writes acc; 1 reads sol, acc,
+    # writes stg, acc; 2 reads stg, sol, acc, writes stg, sol; 3 reads stg, sol, writes sol
+    state = {0: 2 * S, 1: 4 * S, 2: 5 * S, 3: 3 * S}[stage]
     if fam in ("z_forward_stage", "y_forward_stage"):
-        # read F; RK4 stage 0: read sol, write stg+acc; 1,2: read sol,stg,acc, write stg,acc;
-        # 3: read stg,acc,sol, write sol
         return S + state
     if fam in ("z_forward_stage_inverse", "y_forward_stage_inverse"):
         # as above, then the next stage's first inverse pass from registers: write 2
@@ -65,6 +69,74 @@ def kernel_bytes(name: str, nS: int, nR: int, cb: int, rb: int, stage: int):
     if fam in ("prologue_z_inverse_deriv", "prologue_y_inverse_deriv"):
         return 3 * S
     return None
+
+
+# kernel-kind name -> (template, MODE, DIR or NIN) of the HIP kernel it instantiates
+_KIND_TEMPLATE = {
+    "col_fft_fwd": ("col_kernel", 0, -1), "col_fft_inv": ("col_kernel", 0, 1),
+    "col_inv_deriv1": ("col_kernel", 1, 1), "col_inv_deriv2": ("col_kernel", 2, 1),
+    "col_fwd_stage": ("col_kernel", 3, -1), "col_flow_synth": ("col_kernel", 4, 1),
+    "col_fwd_stage_inv": ("col_kernel", 5, -1),
+    "row_c2r": ("row_kernel", 0, 1), "row_r2c": ("row_kernel", 1, 1),
+    "row_advect1": ("row_kernel", 2, 1), "row_advect2": ("row_kernel", 2, 2),
+    "row_advect3": ("row_kernel", 2, 3),
+}
+
+
+def kernel_template(variant: str) -> str:
+    """'col_fwd_stage_inv N=256 R=8 W=2 occ=3 f64' -> 'col_kernel<double, 256, 8, 2, 5, -1, 3>',
+    the kernel name the profiler reports for that variant."""
+    parts = variant.split()
+    kv = dict(p.split("=") for p in parts if "=" in p)
+    tmpl, mode, last = _KIND_TEMPLATE[parts[0]]
+    ty = "double" if parts[-1] == "f64" else "float"
+    return (f"{tmpl}<{ty}, {kv['N']}, {kv['R']}, {kv['W']}, {mode}, {last}, "
+            f"{kv.get('occ', '0')}>")
+
+
+def _counter_rows(d):
+    """(kernel name, counter, value) of every dispatch in a rocprofv3 --pmc csv output folder."""
+    import csv
+    import glob
+    out = []
+    for path in glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True):
+        with open(path, newline="") as f:
+            for r in csv.DictReader(f):
+                out.append((r["Kernel_Name"], r["Counter_Name"], float(r["Counter_Value"])))
+    return out
+
+
+def pmc_summary(fetch_dir, write_dir, cal_fetch_dir, cal_write_dir, bench_json):
+    """HBM bytes per launch of each kernel family of the headline run, from rocprofv3
+    FETCH_SIZE and WRITE_SIZE (KiB per dispatch), each scaled by what the same counter
+    reads for a plain 1 GiB device copy (bench.py --copy-calibration)."""
+    def avg(rows, counter, pred):
+        v = [x for (k, c, x) in rows if c == counter and pred(k)]
+        return (sum(v) / len(v), len(v)) if v else (None, 0)
+    calib = {}
+    for counter, d in (("FETCH_SIZE", cal_fetch_dir), ("WRITE_SIZE", cal_write_dir)):
+        kib, n = avg(_counter_rows(d), counter, lambda k: "copyBuffer" in k)
+        calib[counter] = dict(kib_per_launch=kib, launches=n,
+                              factor=(1 << 30) / (kib * 1024.0) if kib else None)
+    bj = json.loads(open(bench_json).read().strip().splitlines()[-1])
+    rows = {c: _counter_rows(d) for c, d in (("FETCH_SIZE", fetch_dir), ("WRITE_SIZE", write_dir))}
+    fams = {}
+    for fam, k in bj["kernels"].items():
+        tname = kernel_template(k["kernel"])
+        ent = dict(kernel=tname, algorithmic_bytes_per_launch=k.get("bytes_per_launch"))
+        tot = 0.0
+        for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+            kib, n = avg(rows[counter], counter, lambda name: tname in name)
+            fac = calib[counter]["factor"]
+            ent[counter] = dict(kib_per_launch=kib, launches=n)
+            if kib is None or fac is None:
+                tot = None
+            elif tot is not None:
+                tot += kib * 1024.0 * fac
+        ent["hbm_bytes_per_launch"] = tot
+        fams[fam] = ent
+    return dict(calibration=calib, families=fams,
+                per_launch_bytes={f: e["hbm_bytes_per_launch"] for f, e in fams.items()})
 
 
 def gps(points, steps, seconds):
@@ -105,10 +177,11 @@ def headline(dev, steps, warmup, T=np.float64, autotune=True):
     return cfg, prob, secs, (m0, m1, v0, v1)
 
 
-def slot_table(prob, nS, nR, cb, rb, reps=20):
+def slot_table(prob, nS, nR, cb, rb, steps):
+    """Per-launch durations measured inside `steps` running steps (a HIP event around
+    every launch on the problem's stream), with each launch's algorithmic bytes."""
     rows = []
-    for i, name in enumerate(prob.slots()):
-        ms = prob.time_slot(i, reps)
+    for i, (name, ms) in enumerate(zip(prob.slots(), prob.step_profiled(steps))):
         stage = int(name[5]) - 1 if name.startswith("stage") else 0
         b = kernel_bytes(name, nS, nR, cb, rb, stage)
         rows.append(dict(slot=i, name=name, ms=ms, bytes=b,
@@ -236,23 +309,44 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--ensemble-steps", type=int, default=1000)
+    ap.add_argument("--pmc-summary", nargs=5,
+                    metavar=("FETCH_DIR", "WRITE_DIR", "CAL_FETCH_DIR", "CAL_WRITE_DIR", "BENCH_OUT"),
+                    help="summarise rocprofv3 --pmc csv outputs into per-launch HBM bytes (JSON) "
+                         "and exit; runs on the CPU")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain device copy of 1 GiB five times (the reference "
+                         "kernel for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE) and exit")
     args = ap.parse_args()
+    if args.pmc_summary:
+        print(json.dumps(pmc_summary(*args.pmc_summary), indent=1))
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build runs a single problem (or ensemble) per GPU "
               "and has no multi-GPU path; use --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
     import torch  # noqa: F401  (device plumbing: synchronize)
+    if args.copy_calibration:
+        n = 1 << 27                              # 2^27 doubles = 1 GiB
+        a = torch.ones(n, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes_read": 8 * n, "copy_calibration_bytes_written": 8 * n,
+                          "launches": 5}), flush=True)
+        return
     import passivetracerflows as ptf
     dev = ptf.ROCm(0)
 
     cfg, prob, secs, (m0, m1, v0, v1) = headline(dev, args.steps, args.warmup,
                                                  autotune=not args.no_autotune)
+    s64 = prob.sol            # state after warmup + steps, for the mixed-form comparison
     l1 = float(np.abs(cfg["c0"]).sum())
     npts = 256 ** 3
     value = gps(npts, args.steps, secs)
     nS, nR = 129 * 256 * 256, npts
-    rows = slot_table(prob, nS, nR, 16, 8)
+    rows = slot_table(prob, nS, nR, 16, 8, args.steps)
     dname, fam, avg_ms, avg_bytes, fams = dominant(rows)
     achieved = avg_bytes / (avg_ms * 1e-3) / 1e9
     step_bytes = sum(r["bytes"] for r in rows if r["bytes"])
@@ -268,7 +362,7 @@ def main():
     # mixed form of the same workload + its error against the FP64 run
     _, pm, secs32, _ = headline(dev, args.steps, args.warmup, T=np.float32,
                                 autotune=not args.no_autotune)
-    s64, s32 = prob.sol, pm.sol
+    s32 = pm.sol
     mixed_err = float(np.linalg.norm(s32 - s64) / np.linalg.norm(s64))
 
     forms = {
@@ -276,8 +370,8 @@ def main():
                                kernel_ms_per_step=step_ms_kernels,
                                schedule_bytes_per_step=step_bytes,
                                schedule_GBs=step_bytes / (secs / args.steps) / 1e9,
-                               survey_pass_model_GBs=716.8 * value / 1e9,
-                               survey_pass_model_frac=716.8 * value / 1e9 / HBM_PEAK_GBS,
+                               pass_model_GBs=PASS_MODEL_B_PER_GPS * value / 1e9,
+                               pass_model_frac=PASS_MODEL_B_PER_GPS * value / 1e9 / HBM_PEAK_GBS,
                                mass_drift_rel_to_l1=abs(m1 - m0) / l1,
                                variance_ratio=v1 / v0),
         "abc3d_256_mixed": dict(value=gps(npts, args.steps, secs32),
@@ -313,7 +407,8 @@ def main():
                      "kernel": f"{dname} [{fam['kernel']}]", "avg_launch_ms": avg_ms,
                      "algorithmic_bytes_per_launch": avg_bytes},
         "cpu_baseline": cpu,
-        "kernels": {k: dict(ms_per_launch=f["total_ms"] / f["n"], launches_per_step=f["n"],
+        "kernels": {k: dict(kernel=f["kernel"], ms_per_launch=f["total_ms"] / f["n"],
+                            launches_per_step=f["n"], bytes_per_launch=f["bytes"] / f["n"],
                             GBs=(f["bytes"] / f["total_ms"] * 1e3 / 1e9) if f["bytes"] else None)
                     for k, f in fams.items()},
         "forms": forms,

@@ -133,6 +133,11 @@ int ptf_autotune(ptf_problem* prob);
 int ptf_problem_num_slots(ptf_problem* prob, int32_t* nslots);
 int ptf_problem_slot_name(ptf_problem* prob, int32_t slot, char* buf, int32_t buflen);
 int ptf_problem_time_slot(ptf_problem* prob, int32_t slot, int32_t reps, double* avg_ms);
+/* ptf_step with a HIP event recorded around every launch of every step:
+ * slot_ms[i] (num_slots entries) receives the average duration of slot i over
+ * the nsteps steps, measured inside the running step sequence.  The state
+ * advances exactly as with ptf_step. */
+int ptf_step_profiled(ptf_problem* prob, int64_t nsteps, double* slot_ms);
 
 /* ---- ensembles: many independent 2D problems, one graph-captured launch
  *      sequence per step (no reference counterpart: a batch of

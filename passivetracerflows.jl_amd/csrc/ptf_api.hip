@@ -441,6 +441,14 @@ static StageArgs<T> stage_args(const Group& g, int stage) {
   return st;
 }
 
+// whether a stage epilogue writes sol (RK4 stages 3 and 4 in the FP64 form; the last
+// stage otherwise)
+template <typename T>
+static bool writes_sol(const StageArgs<T>& st) {
+  if (st.kind == EPI_STORE) return false;
+  return st.stage == 3 || (st.kind == EPI_RK4 && st.stage == 2);
+}
+
 // Launch list of one calcN! evaluation of `in`, ending in `st` (a stage
 // epilogue or a plain store to `out`).
 // With fused = true the first inverse pass is left out (the previous stage's
@@ -464,7 +472,7 @@ static int build_calcN(Group& g, const cpx<T>* in, const StageArgs<T>& st, cpx<T
     a.st = st;
     set_args(s, a);
     s.name = prefix + "x_advect_stage";
-    s.writes_state = st.kind != EPI_STORE && st.stage == 3;
+    s.writes_state = writes_sol(st);
     CHK(pick_default(s));
     list.push_back(s);
     return PTF_OK;
@@ -503,7 +511,7 @@ static int build_calcN(Group& g, const cpx<T>* in, const StageArgs<T>& st, cpx<T
     a3.st = st;
     set_args(s3, a3);
     s3.name = prefix + (fused ? "y_forward_stage_inverse" : "y_forward_stage");
-    s3.writes_state = st.kind != EPI_STORE && st.stage == 3;
+    s3.writes_state = writes_sol(st);
     CHK(pick_default(s3));
     list.push_back(s3);
     return PTF_OK;
@@ -565,7 +573,7 @@ static int build_calcN(Group& g, const cpx<T>* in, const StageArgs<T>& st, cpx<T
   a5.st = st;
   set_args(s5, a5);
   s5.name = prefix + (fused ? "z_forward_stage_inverse" : "z_forward_stage");
-  s5.writes_state = st.kind != EPI_STORE && st.stage == 3;
+  s5.writes_state = writes_sol(st);
   CHK(pick_default(s5));
   list.push_back(s5);
   return PTF_OK;
@@ -582,6 +590,8 @@ static int build_group_slots(Group& g) {
       in = (const cpx<T>*)(s == 0 ? g.sol.p : s == 1 ? g.stg.p : g.stg2.p);
     }
     StageArgs<T> st = stage_args<T>(g, s);
+    // the fused forms (ndim >= 2) hand the next stage input on in registers
+    st.store_x = g.ndim >= 2 ? 0 : 1;
     CHK(build_calcN<T>(g, in, st, nullptr, g.stages[s], "stage" + std::to_string(s + 1) + "_",
                        g.ndim >= 2));
   }
@@ -1333,6 +1343,60 @@ int ptf_step(ptf_problem* prob, int64_t nsteps) {
   }
   HIPCHK(hipGetLastError());
   return PTF_OK;
+}
+
+int ptf_step_profiled(ptf_problem* prob, int64_t nsteps, double* slot_ms) {
+  if (!prob || !slot_ms) return fail(PTF_ERR_INVALID, "NULL argument");
+  if (nsteps < 1) return fail(PTF_ERR_INVALID, "nsteps must be >= 1");
+  Group& g = prob->g;
+  hipStream_t st = prob->stream;
+  const std::vector<Slot*> v = step_slots(g);
+  const size_t ns = v.size();
+  CHK(group_prime(g, st));
+  // one event before the first launch and one after every launch of every step
+  std::vector<hipEvent_t> ev((size_t)nsteps * ns + 1, nullptr);
+  int rc = PTF_OK;
+  for (auto& e : ev) {
+    if (hipEventCreate(&e) != hipSuccess) {
+      e = nullptr;
+      rc = fail(PTF_ERR_DEVICE, "hipEventCreate failed");
+      break;
+    }
+  }
+  size_t k = 0;
+  if (rc == PTF_OK && hipEventRecord(ev[k++], st) != hipSuccess) rc = fail(PTF_ERR_DEVICE, "hipEventRecord");
+  for (int64_t i = 0; i < nsteps && rc == PTF_OK; ++i) {
+    if (!g.flowsyn.empty()) {
+      Slot& s0 = g.flowsyn[0];
+      if (g.dbl) args_of<ColArgs<double>>(s0).t = prob->t;
+      else args_of<ColArgs<float>>(s0).t = prob->t;
+    }
+    for (Slot* s : v) {
+      if ((rc = launch(*s, st)) != PTF_OK) break;
+      if (hipEventRecord(ev[k++], st) != hipSuccess) {
+        rc = fail(PTF_ERR_DEVICE, "hipEventRecord");
+        break;
+      }
+    }
+    if (rc != PTF_OK) break;
+    prob->t += g.dt;
+    prob->step += 1;
+  }
+  if (rc == PTF_OK && hipEventSynchronize(ev[k - 1]) != hipSuccess)
+    rc = fail(PTF_ERR_DEVICE, "hipEventSynchronize");
+  if (rc == PTF_OK) {
+    for (size_t j = 0; j < ns; ++j) slot_ms[j] = 0.0;
+    for (size_t e = 1; e < k && rc == PTF_OK; ++e) {
+      float ms = 0;
+      if (hipEventElapsedTime(&ms, ev[e - 1], ev[e]) != hipSuccess)
+        rc = fail(PTF_ERR_DEVICE, "hipEventElapsedTime");
+      slot_ms[(e - 1) % ns] += ms;
+    }
+    for (size_t j = 0; j < ns; ++j) slot_ms[j] /= (double)nsteps;
+  }
+  for (auto& e : ev)
+    if (e) (void)hipEventDestroy(e);
+  return rc;
 }
 
 int ptf_get_clock(ptf_problem* prob, double* t, int64_t* step) {

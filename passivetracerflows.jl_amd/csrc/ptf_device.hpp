@@ -268,6 +268,8 @@ template <typename T>
 struct StageArgs {
   int kind;     // EPI_*
   int stage;    // 0..3
+  int store_x;  // RK4 stage 0: also store the next stage input in stg (when the next
+                // stage's first transform reads it from memory)
   cpx<T>* sol;  // state
   cpx<T>* sol_lo;  // mixed form: low part of the compensated state (sol + sol_lo), else NULL
   cpx<T>* stg;  // RK4: stage input sol1;  ETDRK4: sol1
@@ -305,7 +307,16 @@ __device__ __forceinline__ void comp_add(T hi, T lo, T y, T& ho, T& lo_out) {
 // given the nonlinear term N of the stage input.
 // RK4 ([FF] RK4substeps!/RK4update!): RHS_s = N_s + L x_s;
 //   x_2 = sol + dt/2 RHS_1, x_3 = sol + dt/2 RHS_2, x_4 = sol + dt RHS_3,
-//   sol += dt (RHS_1/6 + RHS_2/3 + RHS_3/3 + RHS_4/6)   (sum in this order)
+//   sol += dt (RHS_1/6 + RHS_2/3 + RHS_3/3 + RHS_4/6)   (RHS sum in this order)
+// Storage between stages (acc, stg, sol):
+//   stage 1: acc = RHS_1 (x_2 is recomputed from sol and acc in stage 2, with the
+//            same expression, so it is not stored unless store_x is set);
+//   stage 2: acc = RHS_1/6 + RHS_2/3, stg = x_3;
+//   stage 3: stg = x_4; FP64 form: sol = sol + dt (RHS_1/6 + RHS_2/3 + RHS_3/3),
+//            mixed form: acc = RHS_1/6 + RHS_2/3 + RHS_3/3 (sol stays compensated);
+//   stage 4: FP64 form: sol = sol + dt RHS_4/6; mixed form: (sol, sol_lo) +=
+//            dt (acc + RHS_4/6) with a compensated sum.
+// The FP64 form thus adds the last term to sol in two roundings instead of one.
 // ETDRK4 ([FF] ETDRK4substeps!/ETDRK4update!):
 //   sol1 = E2 sol + zeta N1 ; sol2 = E2 sol + zeta N2 ; sol2 = E2 sol1 + zeta (2 N3 - N1);
 //   sol = E sol + alpha N1 + 2 beta (N2 + N3) + Gamma N4
@@ -326,10 +337,13 @@ __device__ __forceinline__ Rk4Ops<T> rk4_load(const StageArgs<T>& a, long long u
   Rk4Ops<T> o = {z, z, z, z};
   if (a.stage == 0) {
     o.x = (a.sol + ubase)[off];
+  } else if (a.stage == 1) {
+    o.s = (a.sol + ubase)[off];
+    o.ac = (a.acc + ubase)[off];
   } else {
     o.x = (a.stg + ubase)[off];
     o.s = (a.sol + ubase)[off];
-    o.ac = (a.acc + ubase)[off];
+    if (a.stage == 2 || a.sol_lo) o.ac = (a.acc + ubase)[off];
     if (a.stage == 3 && a.sol_lo) o.lo = (a.sol_lo + ubase)[off];
   }
   return o;
@@ -341,34 +355,42 @@ __device__ __forceinline__ cpx<T> rk4_finish(const StageArgs<T>& a, long long ub
                                              const Rk4Ops<T>& o) {
   const T L = (T)linear_op(a.g, a.op[member], a.nkappah, ikx, iky, ikz);
   const T dt = (T)a.dt;
-  const cpx<T> x = o.x;
+  const T h2 = (T)(a.dt / 2);
+  // stage 2 input x_2 = sol + dt/2 RHS_1, recomputed exactly as stage 1 formed it
+  const cpx<T> x = a.stage == 1 ? cpx<T>{o.s.re + h2 * o.ac.re, o.s.im + h2 * o.ac.im} : o.x;
   const cpx<T> rhs = {N.re + L * x.re, N.im + L * x.im};
   if (a.stage == 0) {
-    const T h = (T)(a.dt / 2);
-    const cpx<T> nx = {x.re + h * rhs.re, x.im + h * rhs.im};
-    (a.stg + ubase)[off] = nx;
-    (a.acc + ubase)[off] = {rhs.re / (T)6, rhs.im / (T)6};
+    const cpx<T> nx = {x.re + h2 * rhs.re, x.im + h2 * rhs.im};
+    if (a.store_x) (a.stg + ubase)[off] = nx;
+    (a.acc + ubase)[off] = rhs;
     return nx;
-  } else if (a.stage == 1 || a.stage == 2) {
-    const T h = a.stage == 1 ? (T)(a.dt / 2) : dt;
-    const cpx<T> nx = {o.s.re + h * rhs.re, o.s.im + h * rhs.im};
+  } else if (a.stage == 1) {
+    const cpx<T> nx = {o.s.re + h2 * rhs.re, o.s.im + h2 * rhs.im};
     (a.stg + ubase)[off] = nx;
-    (a.acc + ubase)[off] = {o.ac.re + rhs.re / (T)3, o.ac.im + rhs.im / (T)3};
+    (a.acc + ubase)[off] = {o.ac.re / (T)6 + rhs.re / (T)3, o.ac.im / (T)6 + rhs.im / (T)3};
+    return nx;
+  } else if (a.stage == 2) {
+    const cpx<T> nx = {o.s.re + dt * rhs.re, o.s.im + dt * rhs.im};
+    (a.stg + ubase)[off] = nx;
+    const cpx<T> tot = {o.ac.re + rhs.re / (T)3, o.ac.im + rhs.im / (T)3};
+    if (a.sol_lo) (a.acc + ubase)[off] = tot;
+    else (a.sol + ubase)[off] = {o.s.re + dt * tot.re, o.s.im + dt * tot.im};
     return nx;
   } else {
-    const cpx<T> tot = {o.ac.re + rhs.re / (T)6, o.ac.im + rhs.im / (T)6};
-    if (a.sol_lo) {
-      // compensated accumulation: (hi, lo) += dt tot with TwoSum + renormalisation
-      T hr, lr, hi_, li_;
-      comp_add(o.s.re, o.lo.re, dt * tot.re, hr, lr);
-      comp_add(o.s.im, o.lo.im, dt * tot.im, hi_, li_);
-      (a.sol + ubase)[off] = {hr, hi_};
-      (a.sol_lo + ubase)[off] = {lr, li_};
-      return {hr, hi_};
+    if (!a.sol_lo) {
+      // o.s = sol + dt (RHS_1/6 + RHS_2/3 + RHS_3/3), stored by stage 3
+      const cpx<T> ns = {o.s.re + dt * (rhs.re / (T)6), o.s.im + dt * (rhs.im / (T)6)};
+      (a.sol + ubase)[off] = ns;
+      return ns;
     }
-    const cpx<T> ns = {o.s.re + dt * tot.re, o.s.im + dt * tot.im};
-    (a.sol + ubase)[off] = ns;
-    return ns;
+    // compensated accumulation: (hi, lo) += dt tot with TwoSum + renormalisation
+    const cpx<T> tot = {o.ac.re + rhs.re / (T)6, o.ac.im + rhs.im / (T)6};
+    T hr, lr, hi_, li_;
+    comp_add(o.s.re, o.lo.re, dt * tot.re, hr, lr);
+    comp_add(o.s.im, o.lo.im, dt * tot.im, hi_, li_);
+    (a.sol + ubase)[off] = {hr, hi_};
+    (a.sol_lo + ubase)[off] = {lr, li_};
+    return {hr, hi_};
   }
 }
 

@@ -316,6 +316,16 @@ class Problem:
         call("ptf_problem_time_slot", self._h, i, reps, C.byref(ms))
         return ms.value
 
+    def step_profiled(self, nsteps):
+        """Advance nsteps steps (as stepforward_) with a HIP event around every launch;
+        returns the average duration in ms of each launch slot over those steps."""
+        n = C.c_int32()
+        call("ptf_problem_num_slots", self._h, C.byref(n))
+        out = (C.c_double * n.value)()
+        call("ptf_step_profiled", self._h, int(nsteps), out)
+        self._sync_clock()
+        return list(out)
+
     def __del__(self):
         try:
             if getattr(self, "_h", None):
