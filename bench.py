@@ -139,6 +139,35 @@ def pmc_summary(fetch_dir, write_dir, cal_fetch_dir, cal_write_dir, bench_json):
                 per_launch_bytes={f: e["hbm_bytes_per_launch"] for f, e in fams.items()})
 
 
+def trace_summary(trace_dir, bench_out, steps, warmup):
+    """Markdown summary of a rocprofv3 --kernel-trace --output-format csv run of bench.py:
+    for each kernel family of the FP64 headline, the average duration of the chosen
+    variant over the dispatches of the running steps (warmup + timed + profiled steps, the
+    last dispatches of that kernel in the FP64 part of the run), beside bench.py's own
+    HIP-event figure."""
+    import csv
+    import glob
+    bj = json.loads(open(bench_out).read().strip().splitlines()[-1])
+    disp = []
+    for path in glob.glob(os.path.join(trace_dir, "**", "*kernel_trace.csv"), recursive=True):
+        with open(path, newline="") as f:
+            for r in csv.DictReader(f):
+                disp.append((int(r["Start_Timestamp"]), int(r["End_Timestamp"]), r["Kernel_Name"]))
+    disp.sort()
+    lines = ["| family | kernel | launches/step | rocprof avg us (in-step dispatches) | n | "
+             "bench.py HIP-event avg us | ratio |", "|---|---|---|---|---|---|---|"]
+    for fam, k in bj["kernels"].items():
+        tname = kernel_template(k["kernel"])
+        d = [(e - b) for (b, e, n) in disp if tname in n]
+        nstep = k["launches_per_step"] * (warmup + 2 * steps)
+        use = d[-nstep:] if len(d) >= nstep else d
+        avg = sum(use) / len(use) / 1e3 if use else float("nan")
+        ev = k["ms_per_launch"] * 1e3
+        lines.append(f"| {fam} | `{tname}` | {k['launches_per_step']} | {avg:.1f} | {len(use)} | "
+                     f"{ev:.1f} | {avg / ev:.3f} |")
+    return "\n".join(lines)
+
+
 def gps(points, steps, seconds):
     return points * steps / seconds
 
@@ -313,12 +342,18 @@ def main():
                     metavar=("FETCH_DIR", "WRITE_DIR", "CAL_FETCH_DIR", "CAL_WRITE_DIR", "BENCH_OUT"),
                     help="summarise rocprofv3 --pmc csv outputs into per-launch HBM bytes (JSON) "
                          "and exit; runs on the CPU")
+    ap.add_argument("--trace-summary", nargs=2, metavar=("TRACE_DIR", "BENCH_OUT"),
+                    help="summarise a rocprofv3 --kernel-trace csv output of this script (run "
+                         "with the same --steps/--warmup) against its JSON line, and exit")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain device copy of 1 GiB five times (the reference "
                          "kernel for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE) and exit")
     args = ap.parse_args()
     if args.pmc_summary:
         print(json.dumps(pmc_summary(*args.pmc_summary), indent=1))
+        return
+    if args.trace_summary:
+        print(trace_summary(*args.trace_summary, args.steps, args.warmup))
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build runs a single problem (or ensemble) per GPU "
