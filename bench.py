@@ -90,6 +90,8 @@ def kernel_template(variant: str) -> str:
     kv = dict(p.split("=") for p in parts if "=" in p)
     tmpl, mode, last = _KIND_TEMPLATE[parts[0]]
     ty = "double" if parts[-1] == "f64" else "float"
+    if "par" in parts:   # input-parallel advection: row_adv_par_kernel<T, NX, R, NL, NIN, OCC>
+        return f"row_adv_par_kernel<{ty}, {kv['N']}, {kv['R']}, {kv['W']}, {last}, {kv.get('occ', '0')}>"
     return (f"{tmpl}<{ty}, {kv['N']}, {kv['R']}, {kv['W']}, {mode}, {last}, "
             f"{kv.get('occ', '0')}>")
 

@@ -133,6 +133,12 @@ int ptf_autotune(ptf_problem* prob);
 int ptf_problem_num_slots(ptf_problem* prob, int32_t* nslots);
 int ptf_problem_slot_name(ptf_problem* prob, int32_t slot, char* buf, int32_t buflen);
 int ptf_problem_time_slot(ptf_problem* prob, int32_t slot, int32_t reps, double* avg_ms);
+/* Kernel variant of slot i: radix R, width W (columns or rows per block) and
+ * flags (PTF_VARIANT_*).  set_variant selects any compiled variant (as the
+ * autotuner does); PTF_ERR_UNSUPPORTED when that variant was not compiled. */
+enum { PTF_VARIANT_OCC3 = 1, PTF_VARIANT_PAR = 2 };
+int ptf_problem_set_variant(ptf_problem* prob, int32_t slot, int32_t R, int32_t W, int32_t flags);
+int ptf_problem_get_variant(ptf_problem* prob, int32_t slot, int32_t* R, int32_t* W, int32_t* flags);
 /* ptf_step with a HIP event recorded around every launch of every step:
  * slot_ms[i] (num_slots entries) receives the average duration of slot i over
  * the nsteps steps, measured inside the running step sequence.  The state

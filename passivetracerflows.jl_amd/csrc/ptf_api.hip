@@ -106,6 +106,7 @@ struct Slot {
   int N = 0;            // axis length (columns) or nx (rows)
   int R = 0, W = 0;     // radix, columns/rows per block
   int occ = 0;          // 0: default register budget; 3: 3 waves/SIMD
+  int par = 0;          // 1: input-parallel advection (row_adv_par_kernel)
   const void* fn = nullptr;
   long long lines = 0;  // ncols (columns) or nrows (rows)
   int other = 1;        // grid.y for columns
@@ -116,27 +117,33 @@ struct Slot {
 
   bool is_col() const { return kind < 10; }
   bool has_occ() const { return kind == 3 || kind == 6 || kind == 13 || kind == 14; }
-  int threads() const { return is_col() ? W * (N / R) : W * ((N / 2) / R); }
+  bool has_par() const { return kind == 13 || kind == 14; }
+  int threads() const {
+    if (is_col()) return W * (N / R);
+    return (par ? (kind == 14 ? 3 : 2) : 1) * W * ((N / 2) / R);
+  }
   dim3 grid() const {
     if (is_col()) return dim3((unsigned)((lines + W - 1) / W), (unsigned)other, (unsigned)members);
     return dim3((unsigned)((lines + W - 1) / W), (unsigned)members, 1);
   }
 };
 
-static bool choose_variant(Slot& s, int R, int W, int occ = 0) {
+static bool choose_variant(Slot& s, int R, int W, int occ = 0, int par = 0) {
   RegFn f = reg_fn(s.dbl, s.kind);
   if (!f) return false;
   if (occ && !s.has_occ()) return false;
+  if (par && !s.has_par()) return false;
   if (s.brick > 0) {
     // a bricked line needs whole bricks per thread group; bricked rows whole bricks per block
     if (s.is_col() && (R > s.N || (s.N / R) % (1 << s.brick) != 0)) return false;
     if (!s.is_col() && W % (1 << s.brick) != 0) return false;
   }
-  const void* p = f(s.N, R, W + 1000 * occ);
+  const void* p = f(s.N, R, W + 100 * par + 1000 * occ);
   if (!p) return false;
   s.R = R;
   s.W = W;
   s.occ = occ;
+  s.par = par;
   s.fn = p;
   return true;
 }
@@ -1022,21 +1029,25 @@ static int autotune_group(Group& g, hipStream_t st) {
     int bestR = s->R, bestW = s->W, bestOcc = s->occ;
     double best = 1e30;
     const bool verbose = std::getenv("PTF_AUTOTUNE_VERBOSE") != nullptr;
-    for (int occ : {0, 3}) {
-      for (int R : kRadices) {
-        for (int wi = 0; wi < nw; ++wi) {
-          Slot trial = *s;
-          if (!choose_variant(trial, R, widths[wi], occ)) continue;
-          double ms = 0;
-          CHK(time_slot(g, trial, 5, st, &ms));
-          if (verbose)
-            std::fprintf(stderr, "autotune %s N=%d R=%d W=%d occ=%d: %.4f ms\n",
-                         kind_name(trial.kind), trial.N, R, widths[wi], occ, ms);
-          if (ms < best) {
-            best = ms;
-            bestR = R;
-            bestW = widths[wi];
-            bestOcc = occ;
+    int bestPar = s->par;
+    for (int par : {0, 1}) {
+      for (int occ : {0, 3}) {
+        for (int R : kRadices) {
+          for (int wi = 0; wi < nw; ++wi) {
+            Slot trial = *s;
+            if (!choose_variant(trial, R, widths[wi], occ, par)) continue;
+            double ms = 0;
+            CHK(time_slot(g, trial, 5, st, &ms));
+            if (verbose)
+              std::fprintf(stderr, "autotune %s N=%d R=%d W=%d occ=%d par=%d: %.4f ms\n",
+                           kind_name(trial.kind), trial.N, R, widths[wi], occ, par, ms);
+            if (ms < best) {
+              best = ms;
+              bestR = R;
+              bestW = widths[wi];
+              bestOcc = occ;
+              bestPar = par;
+            }
           }
         }
       }
@@ -1044,7 +1055,7 @@ static int autotune_group(Group& g, hipStream_t st) {
     for (Slot* o : all) {
       const std::string k2 = std::to_string(o->kind) + ":" + std::to_string(o->N) + ":" +
                              std::to_string(o->lines) + ":" + std::to_string(o->members);
-      if (k2 == key) choose_variant(*o, bestR, bestW, bestOcc);
+      if (k2 == key) choose_variant(*o, bestR, bestW, bestOcc, bestPar);
     }
   }
   return PTF_OK;
@@ -1551,8 +1562,9 @@ int ptf_problem_num_slots(ptf_problem* prob, int32_t* n) {
 static int slot_name_into(Slot& s, char* buf, int32_t buflen) {
   if (!buf || buflen <= 0) return fail(PTF_ERR_INVALID, "bad buffer");
   char tmp[256];
-  std::snprintf(tmp, sizeof(tmp), "%s [%s N=%d R=%d W=%d%s %s]", s.name.c_str(),
-                kind_name(s.kind), s.N, s.R, s.W, s.occ ? " occ=3" : "", s.dbl ? "f64" : "f32");
+  std::snprintf(tmp, sizeof(tmp), "%s [%s N=%d R=%d W=%d%s%s %s]", s.name.c_str(),
+                kind_name(s.kind), s.N, s.R, s.W, s.occ ? " occ=3" : "", s.par ? " par" : "",
+                s.dbl ? "f64" : "f32");
   std::strncpy(buf, tmp, (size_t)buflen - 1);
   buf[buflen - 1] = 0;
   return PTF_OK;
@@ -1570,6 +1582,31 @@ int ptf_problem_time_slot(ptf_problem* prob, int32_t slot, int32_t reps, double*
   auto v = step_slots(prob->g);
   if (slot < 0 || slot >= (int)v.size()) return fail(PTF_ERR_INVALID, "slot out of range");
   return time_slot(prob->g, *v[slot], reps, prob->stream, avg_ms);
+}
+
+int ptf_problem_set_variant(ptf_problem* prob, int32_t slot, int32_t R, int32_t W, int32_t flags) {
+  if (!prob) return fail(PTF_ERR_INVALID, "problem is NULL");
+  auto v = step_slots(prob->g);
+  if (slot < 0 || slot >= (int)v.size()) return fail(PTF_ERR_INVALID, "slot out of range");
+  if (flags & ~(PTF_VARIANT_OCC3 | PTF_VARIANT_PAR)) return fail(PTF_ERR_INVALID, "unknown flags");
+  Slot trial = *v[slot];
+  if (!choose_variant(trial, R, W, (flags & PTF_VARIANT_OCC3) ? 3 : 0,
+                      (flags & PTF_VARIANT_PAR) ? 1 : 0))
+    return fail(PTF_ERR_UNSUPPORTED, "no compiled variant R=" + std::to_string(R) +
+                                         " W=" + std::to_string(W) + " flags=" +
+                                         std::to_string(flags) + " for slot " + trial.name);
+  *v[slot] = trial;
+  return PTF_OK;
+}
+
+int ptf_problem_get_variant(ptf_problem* prob, int32_t slot, int32_t* R, int32_t* W, int32_t* flags) {
+  if (!prob || !R || !W || !flags) return fail(PTF_ERR_INVALID, "NULL argument");
+  auto v = step_slots(prob->g);
+  if (slot < 0 || slot >= (int)v.size()) return fail(PTF_ERR_INVALID, "slot out of range");
+  *R = v[slot]->R;
+  *W = v[slot]->W;
+  *flags = (v[slot]->occ ? PTF_VARIANT_OCC3 : 0) | (v[slot]->par ? PTF_VARIANT_PAR : 0);
+  return PTF_OK;
 }
 
 // ----- ensembles ---------------------------------------------------------------

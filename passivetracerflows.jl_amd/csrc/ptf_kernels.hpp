@@ -554,4 +554,122 @@ void row_kernel(RowArgs<T> a) {
   }
 }
 
+// ROW_ADV (spectral output, no stage epilogue) with the NIN input rows of a block
+// transformed side by side.  The block is NIN groups of NL * M/R threads (whole
+// wavefronts); group d loads its spectral rows and flow values at once and
+// forms the product flow_d * C2R(D_d in_d) for the block's NL rows.  Groups
+// 1.. hand their products over through LDS and end; group 0 forms
+// ((-u cx) - v cy) - w cz and performs the R2C.  A workgroup barrier waits only
+// on the wavefronts that are still running, so group 0 keeps using barriers
+// after the others have ended.  Compared with row_kernel this trades idle lanes
+// in the R2C for fewer registers per lane and all loads of a block in flight
+// together.
+template <typename T, int NX, int R, int NL, int NIN>
+constexpr int adv_par_threads() { return NIN * NL * ((NX / 2) / R); }
+template <typename T, int NX, int R, int NL, int NIN>
+constexpr int adv_par_lds() { return NIN * NL * (NX / 2 + 1) * (int)sizeof(cpx<T>); }
+
+template <typename T, int NX, int R, int NL, int NIN, int OCC = 0>
+__global__ __attribute__((amdgpu_flat_work_group_size(1, adv_par_threads<T, NX, R, NL, NIN>()),
+                          amdgpu_waves_per_eu(
+                              min_waves(adv_par_threads<T, NX, R, NL, NIN>(), adv_par_lds<T, NX, R, NL, NIN>(), OCC),
+                              lds_waves(adv_par_threads<T, NX, R, NL, NIN>(), adv_par_lds<T, NX, R, NL, NIN>()))))
+void row_adv_par_kernel(RowArgs<T> a) {
+  constexpr int M = NX / 2;
+  constexpr int NXH = M + 1;
+  constexpr int P = M / R;
+  constexpr int G = NL * P;   // threads per group
+  static_assert(G % 64 == 0, "a group must be whole wavefronts");
+  static_assert(NIN >= 2 && NIN <= 3, "two or three inputs");
+  using Lay = LayStrided;
+  const Lay lay = a.lb ? Lay{1, NL} : Lay{M + 1, 1};
+  __shared__ cpx<T> lds_all[NIN * NL * (M + 1)];
+  const int grp = (int)threadIdx.x / G;   // uniform over a wavefront
+  const int t = (int)threadIdx.x - grp * G;
+  cpx<T>* lds = lds_all + grp * (NL * (M + 1));
+  const int lb = a.lb;
+  const int rb = t & ((1 << lb) - 1);
+  const int rest = t >> lb;
+  const int j = rest % P;
+  const int g = rest / P;
+  const int c = (g << lb) + rb;
+  const long long row = (long long)blockIdx.x * NL + c;
+  const bool valid = row < a.nrows;
+  const int gmax = (int)((a.nrows - (long long)blockIdx.x * NL) >> lb) - 1;
+  const int gc = valid ? g : (g < gmax ? g : gmax);
+  const int member = blockIdx.y;
+  const long long sub = (long long)member * a.smstride + (long long)blockIdx.x * NL * NXH;
+  const long long pub = (long long)member * a.pmstride + (long long)blockIdx.x * NL * NX;
+  const uint32_t so = ((uint32_t)gc * NXH << lb) + rb;
+  const uint32_t so_st = ((uint32_t)g * NXH << lb) + rb;
+  const uint32_t po = ((uint32_t)gc * M << lb) + rb;
+  const T s = (T)a.scale;
+  // select, not a.in[grp]: a runtime index into the by-value arguments would go to scratch
+  const cpx<T>* in = grp == 0 ? a.in[0] : (grp == 1 ? a.in[1] : a.in[2]);
+  const T* flp = grp == 0 ? a.flow[0] : (grp == 1 ? a.flow[1] : a.flow[2]);
+  cpx<T> z[R], last = {(T)0, (T)0};
+  load_row<T, M, R>(in + sub + so, z, last, j, lb);
+  const cpx<T>* fl = reinterpret_cast<const cpx<T>*>(flp + pub);
+  cpx<T> fv[R];
+#pragma unroll
+  for (int q = 0; q < R; ++q) fv[q] = fl[po + ((uint32_t)(j + q * P) << lb)];
+  cpx<T> zm[R];
+  stage_row_pairs<T, M, R, Lay>(z, last, zm, lds, lay, c, j);
+#pragma unroll
+  for (int q = 0; q < R; ++q) {
+    const int k = j + q * P;
+    cpx<T> Xk = z[q];
+    cpx<T> Xm = zm[q];
+    if (grp == 0) {
+      Xk = times_ik(Xk, (T)(k * a.kscale_x));
+      Xm = times_ik(Xm, (T)((M - k) * a.kscale_x));
+    }
+    z[q] = c2r_pre(Xk, Xm, a.tw[k * a.twstride], s, k == 0);
+  }
+  line_fft<T, M, R, +1, Lay>(z, lds, lay, c, j, a.tw, 2 * a.twstride);
+  cpx<T> f[R];
+#pragma unroll
+  for (int q = 0; q < R; ++q) f[q] = {fv[q].re * z[q].re, fv[q].im * z[q].im};
+  __syncthreads();  // every group's last line_fft reads are done
+  if (grp != 0) {
+#pragma unroll
+    for (int q = 0; q < R; ++q) lds[lay.idx(c, j + q * P)] = f[q];
+  }
+  __syncthreads();
+  if (grp != 0) return;
+#pragma unroll
+  for (int q = 0; q < R; ++q) {
+    const int i = lay.idx(c, j + q * P);
+    cpx<T> v = {-f[q].re, -f[q].im};
+    const cpx<T> p1 = lds_all[NL * (M + 1) + i];
+    v = {v.re - p1.re, v.im - p1.im};
+    if constexpr (NIN == 3) {
+      const cpx<T> p2 = lds_all[2 * NL * (M + 1) + i];
+      v = {v.re - p2.re, v.im - p2.im};
+    }
+    f[q] = v;
+  }
+  // forward half-length FFT of f[2m] + i f[2m+1] (barriers now among group 0 only)
+  line_fft<T, M, R, -1, Lay>(f, lds, lay, c, j, a.tw, 2 * a.twstride);
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < R; ++q) lds[lay.idx(c, j + q * P)] = f[q];
+  __syncthreads();
+  if (valid) {
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+      const int k = j + q * P;
+      cpx<T> X;
+      if (k == 0) {
+        X = {f[q].re + f[q].im, (T)0};
+        (a.out + sub)[so_st + ((uint32_t)M << lb)] = cpx<T>{f[q].re - f[q].im, (T)0};
+      } else {
+        const cpx<T> Zm = lds[lay.idx(c, M - k)];
+        X = r2c_post(f[q], Zm, a.tw[k * a.twstride]);
+      }
+      (a.out + sub)[so_st + ((uint32_t)k << lb)] = X;
+    }
+  }
+}
+
 }  // namespace ptf

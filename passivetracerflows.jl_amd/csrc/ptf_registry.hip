@@ -3,7 +3,8 @@
 // exports  const void* ptf_reg_<TNAME>_<KIND>(int N, int R, int W)  returning
 // the kernel for line length N (column kinds: axis length; row kinds: nx),
 // radix R and width W (columns per block TX, or rows per block NL; plus
-// 1000 * occ for the variants built with a 3-waves/SIMD budget), or NULL
+// 100 for the input-parallel advection form and 1000 * occ for the variants
+// built with a 3-waves/SIMD budget), or NULL
 // when that variant does not exist.  Only variants that satisfy the launch
 // and LDS limits are instantiated.
 #include "ptf_kernels.hpp"
@@ -64,6 +65,41 @@ const void* rowp() {
   }
 }
 
+// input-parallel advection (row_adv_par_kernel), kinds 13 and 14
+template <typename T, int NX, int R, int NL, int KIND, int OCC>
+const void* rowpar() {
+  if constexpr (KIND == K_ROW_ADV2 || KIND == K_ROW_ADV3) {
+    constexpr int NIN = row_nin(KIND);
+    constexpr int M = NX / 2;
+    constexpr int P = M / R;
+    constexpr bool ok = R <= M && R >= 4 && (NL * P) % 64 == 0 && NIN * NL * P <= 1024 &&
+                        (long long)NIN * NL * (M + 1) * (long long)sizeof(cpx<T>) <= 163840;
+    if constexpr (ok) {
+      return reinterpret_cast<const void*>(&row_adv_par_kernel<T, NX, R, NL, NIN, OCC>);
+    } else {
+      return nullptr;
+    }
+  } else {
+    return nullptr;
+  }
+}
+
+template <typename T, int N, int R, int KIND, int OCC>
+const void* by_w_par(int W) {
+  if constexpr (KIND >= 10) {
+    switch (W) {
+      case 1: return rowpar<T, N, R, 1, KIND, OCC>();
+      case 2: return rowpar<T, N, R, 2, KIND, OCC>();
+      case 4: return rowpar<T, N, R, 4, KIND, OCC>();
+      case 8: return rowpar<T, N, R, 8, KIND, OCC>();
+      case 16: return rowpar<T, N, R, 16, KIND, OCC>();
+      default: return nullptr;
+    }
+  } else {
+    return nullptr;
+  }
+}
+
 template <typename T, int N, int R, int KIND, int OCC>
 const void* by_w_occ(int W) {
   if constexpr (KIND < 10) {
@@ -88,10 +124,18 @@ const void* by_w_occ(int W) {
   }
 }
 
-// W encodes the width and the occupancy request: W = width + 1000 * occ
+// W encodes the width, the form and the occupancy request:
+// W = width + 100 * par + 1000 * occ  (par = 1: input-parallel advection)
 template <typename T, int N, int R, int KIND>
 const void* by_w(int W) {
-  const int occ = W / 1000, w = W % 1000;
+  const int occ = W / 1000, par = (W / 100) % 10, w = W % 100;
+  if (par) {
+    if (occ == 0) return by_w_par<T, N, R, KIND, 0>(w);
+    if constexpr (has_occ(KIND)) {
+      if (occ == 3) return by_w_par<T, N, R, KIND, 3>(w);
+    }
+    return nullptr;
+  }
   if (occ == 0) return by_w_occ<T, N, R, KIND, 0>(w);
   if constexpr (has_occ(KIND)) {
     if (occ == 3) return by_w_occ<T, N, R, KIND, 3>(w);

@@ -316,6 +316,16 @@ class Problem:
         call("ptf_problem_time_slot", self._h, i, reps, C.byref(ms))
         return ms.value
 
+    def set_variant(self, i, R, W, occ3=False, par=False):
+        """Select kernel variant (radix R, width W, 3-waves/SIMD build, input-parallel
+        advection) for launch slot i; raises PTFError if it was not compiled."""
+        call("ptf_problem_set_variant", self._h, i, R, W, (1 if occ3 else 0) | (2 if par else 0))
+
+    def get_variant(self, i):
+        R, W, f = C.c_int32(), C.c_int32(), C.c_int32()
+        call("ptf_problem_get_variant", self._h, i, C.byref(R), C.byref(W), C.byref(f))
+        return R.value, W.value, bool(f.value & 1), bool(f.value & 2)
+
     def step_profiled(self, nsteps):
         """Advance nsteps steps (as stepforward_) with a HIP event around every launch;
         returns the average duration in ms of each launch slot over those steps."""
