@@ -122,6 +122,18 @@ int ptf_get_L(ptf_problem* prob, double* L);
 /* FP64 diagnostics: mass = sum(c) = Re sol[0]; variance = mean((c - mean c)^2)
  * by a compensated Parseval sum over the half spectrum. */
 int ptf_diagnostics(ptf_problem* prob, double* mass, double* variance);
+/* stepforward!(prob, diags, nsteps) with the diagnostics kept on the GPU
+ * ([FF] Diagnostic/increment!, used with Output/saveoutput as in
+ * examples/onedim_gaussiandiffusion.jl:66-95): after every step whose new
+ * clock.step is a multiple of `every`, record t, step, mass and variance (as
+ * ptf_diagnostics) and, when c_out != NULL, the physical field c (as
+ * ptf_updatevars) into device memory; everything is copied to the host once,
+ * after the steps.  *nrec_out = number of records; capacity = entries the
+ * output arrays hold (nrec > capacity is an error, raised before any step).
+ * Any output pointer may be NULL; c_out holds nrec R-fields (FP64). */
+int ptf_step_record(ptf_problem* prob, int64_t nsteps, int64_t every, int64_t capacity,
+                    int64_t* nrec_out, double* t_out, int64_t* step_out, double* mass_out,
+                    double* var_out, double* c_out);
 /* Device pointer of sol (S-field of the problem's precision). */
 int ptf_problem_sol_ptr(ptf_problem* prob, void** dptr);
 /* Pick the fastest compiled kernel variant for every launch slot. */

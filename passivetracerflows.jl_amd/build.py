@@ -45,7 +45,7 @@ def _units():
             units.append((obj, cmd, ["ptf_registry.hip", *HEADERS]))
     obj = os.path.join(OBJDIR, "ptf_api.o")
     units.append((obj, [HIPCC, *COMMON, "-c", os.path.join(CSRC, "ptf_api.hip"), "-o", obj],
-                  ["ptf_api.hip", *API_HEADERS]))
+                  ["ptf_api.hip", *API_HEADERS, os.path.join(INCLUDE, "ptf.h")]))
     return units
 
 
@@ -53,8 +53,7 @@ def _stale(obj, deps):
     if not os.path.exists(obj):
         return True
     t = os.path.getmtime(obj)
-    return any(os.path.getmtime(os.path.join(CSRC, d)) > t for d in deps) or \
-        os.path.getmtime(os.path.join(INCLUDE, "ptf.h")) > t
+    return any(os.path.getmtime(os.path.join(CSRC, d)) > t for d in deps)
 
 
 def _run(cmd):

@@ -1542,6 +1542,89 @@ int ptf_diagnostics(ptf_problem* prob, double* mass, double* variance) {
   return diagnostics_group(prob->g, 0, prob->stream, mass, variance);
 }
 
+int ptf_step_record(ptf_problem* prob, int64_t nsteps, int64_t every, int64_t capacity,
+                    int64_t* nrec_out, double* t_out, int64_t* step_out, double* mass_out,
+                    double* var_out, double* c_out) {
+  if (!prob || !nrec_out) return fail(PTF_ERR_INVALID, "NULL argument");
+  if (nsteps < 0 || every < 1) return fail(PTF_ERR_INVALID, "need nsteps >= 0 and every >= 1");
+  Group& g = prob->g;
+  hipStream_t st = prob->stream;
+  // records after the steps whose new clock.step is a multiple of `every`
+  auto floordiv = [](int64_t a, int64_t b) { return a >= 0 ? a / b : -((-a + b - 1) / b); };
+  const int64_t s0 = prob->step;
+  const int64_t nrec = floordiv(s0 + nsteps, every) - floordiv(s0, every);
+  *nrec_out = nrec;
+  if (nrec > capacity)
+    return fail(PTF_ERR_INVALID, "capacity " + std::to_string(capacity) + " < " +
+                                     std::to_string(nrec) + " records");
+  const bool want_diag = mass_out || var_out;
+  DevBuf rec, snaps;   // freed on return (hipFree waits for the device)
+  const size_t rsz = (size_t)g.nR * g.esz;   // one physical snapshot
+  int rc = PTF_OK;
+  if (want_diag && nrec > 0) rc = dalloc(rec, (size_t)nrec * 2 * sizeof(double));
+  if (rc == PTF_OK && c_out && nrec > 0) rc = dalloc(snaps, (size_t)nrec * rsz);
+  if (rc != PTF_OK) return rc;
+  const int blocks = 1024;
+  int64_t r = 0;
+  for (int64_t i = 0; i < nsteps && rc == PTF_OK; ++i) {
+    rc = group_step_launches(g, st, prob->t);
+    if (rc != PTF_OK) break;
+    prob->t += g.dt;
+    prob->step += 1;
+    if (prob->step % every != 0) continue;
+    if (t_out) t_out[r] = prob->t;
+    if (step_out) step_out[r] = prob->step;
+    if (want_diag) {
+      double* dst = (double*)rec.p + 2 * r;
+      if (g.dbl) {
+        hipLaunchKernelGGL(variance_partial_kernel<double>, dim3(blocks), dim3(256), 0, st,
+                           (const cpx<double>*)g.sol.p, g.nS, g.nxh, g.lb, (double*)g.red.p);
+        hipLaunchKernelGGL(diag_record_kernel<double>, dim3(1), dim3(64), 0, st,
+                           (const cpx<double>*)g.sol.p, (const double*)g.red.p, 2 * blocks,
+                           (double)g.nR, dst);
+      } else {
+        hipLaunchKernelGGL(variance_partial_kernel<float>, dim3(blocks), dim3(256), 0, st,
+                           (const cpx<float>*)g.sol.p, g.nS, g.nxh, g.lb, (double*)g.red.p);
+        hipLaunchKernelGGL(diag_record_kernel<float>, dim3(1), dim3(64), 0, st,
+                           (const cpx<float>*)g.sol.p, (const double*)g.red.p, 2 * blocks,
+                           (double)g.nR, dst);
+      }
+      if (hipGetLastError() != hipSuccess) rc = fail(PTF_ERR_DEVICE, "diagnostics launch failed");
+    }
+    if (rc == PTF_OK && c_out) rc = g_irfft(g, g.sol.p, (char*)snaps.p + (size_t)r * rsz, st);
+    ++r;
+  }
+  if (rc == PTF_OK && want_diag && nrec > 0) {
+    std::vector<double> h((size_t)nrec * 2);
+    if (hipMemcpyAsync(h.data(), rec.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, st) !=
+            hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess) {
+      rc = fail(PTF_ERR_DEVICE, "download of the diagnostics failed");
+    } else {
+      for (int64_t k = 0; k < nrec; ++k) {
+        if (mass_out) mass_out[k] = h[2 * k];
+        if (var_out) var_out[k] = h[2 * k + 1];
+      }
+    }
+  }
+  if (rc == PTF_OK && c_out) {
+    std::vector<double> tmp(g.lb ? (size_t)g.nR : 0);
+    for (int64_t k = 0; k < nrec && rc == PTF_OK; ++k) {
+      double* dst = c_out + (size_t)k * g.nR;
+      const void* src = (const char*)snaps.p + (size_t)k * rsz;
+      if (!g.lb) {
+        rc = download_real(g, dst, src, (size_t)g.nR, st);
+      } else {
+        rc = download_real(g, tmp.data(), src, (size_t)g.nR, st);
+        if (rc == PTF_OK) from_brick(g, tmp.data(), dst, (size_t)g.nx / 2, 2);
+      }
+    }
+  }
+  if (rc == PTF_OK && hipStreamSynchronize(st) != hipSuccess)
+    rc = fail(PTF_ERR_DEVICE, "hipStreamSynchronize failed");
+  return rc;
+}
+
 int ptf_problem_sol_ptr(ptf_problem* prob, void** dptr) {
   if (!prob || !dptr) return fail(PTF_ERR_INVALID, "NULL argument");
   *dptr = prob->g.sol.p;

@@ -122,4 +122,25 @@ __global__ void variance_partial_kernel(const cpx<T>* __restrict__ sol, long lon
   }
 }
 
+// Mass and variance record from the partial sums of variance_partial_kernel:
+// rec[0] = Re sol[0] (mass = sum of c), rec[1] = variance.  One thread sums the
+// partials in order with Neumaier compensation, as the host does for a single
+// diagnostics call, so both give the same value.
+template <typename T>
+__global__ void diag_record_kernel(const cpx<T>* __restrict__ sol, const double* __restrict__ partial,
+                                   int npartial, double npoints, double* __restrict__ rec) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double sum = 0.0, comp = 0.0;
+  for (int i = 0; i < npartial; ++i) {
+    const double x = partial[i];
+    const double t = sum + x;
+    comp += (fabs(sum) >= fabs(x)) ? (sum - t) + x : (x - t) + sum;
+    sum = t;
+  }
+  const double s0re = (double)sol[0].re, s0im = (double)sol[0].im;
+  const double dc = s0re * s0re + s0im * s0im;
+  rec[0] = s0re;
+  rec[1] = ((sum - dc) + comp) / (npoints * npoints);
+}
+
 }  // namespace ptf

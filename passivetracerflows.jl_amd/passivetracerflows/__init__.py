@@ -9,6 +9,7 @@ from . import _lib
 from . import fourierflows as FourierFlows
 from . import traceradvectiondiffusion as TracerAdvectionDiffusion
 from .ensemble import Ensemble
+from .output import Diagnostic, Output, increment_, jldopen, record_, save_records, saveoutput, saveproblem
 from .fourierflows import (GPU, Clock, DeviceArray, OneDGrid, ROCm, ThreeDGrid, TwoDGrid, gridpoints,
                            ldiv_, mul_, step_until_, stepforward_)
 from .traceradvectiondiffusion import (OneDAdvectingFlow, SpectralStreamfunctionFlow, ThreeDAdvectingFlow,
@@ -20,5 +21,6 @@ __all__ = [
     "FourierFlows", "TracerAdvectionDiffusion", "Ensemble", "ROCm", "GPU", "Clock", "DeviceArray",
     "OneDGrid", "TwoDGrid", "ThreeDGrid", "gridpoints", "mul_", "ldiv_", "stepforward_", "step_until_",
     "OneDAdvectingFlow", "TwoDAdvectingFlow", "ThreeDAdvectingFlow", "SpectralStreamfunctionFlow",
-    "set_c_", "updatevars_",
+    "set_c_", "updatevars_", "Diagnostic", "increment_", "Output", "saveproblem", "saveoutput",
+    "save_records", "record_", "jldopen",
 ]

@@ -89,6 +89,8 @@ SIGNATURES = {
     "ptf_problem_slot_name": [_VP, _I32, C.c_char_p, _I32],
     "ptf_problem_time_slot": [_VP, _I32, _I32, _DP],
     "ptf_step_profiled": [_VP, _I64, _DP],
+    "ptf_step_record": [_VP, _I64, _I64, _I64, C.POINTER(_I64), _DP, C.POINTER(_I64), _DP, _DP,
+                        _DP],
     "ptf_problem_set_variant": [_VP, _I32, _I32, _I32, _I32],
     "ptf_problem_get_variant": [_VP, _I32, C.POINTER(_I32), C.POINTER(_I32), C.POINTER(_I32)],
     "ptf_ensemble_create": [C.POINTER(ptf_desc), _I32, C.POINTER(_VP)],

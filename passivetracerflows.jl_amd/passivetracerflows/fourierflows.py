@@ -345,14 +345,18 @@ class Problem:
             pass
 
 
-def stepforward_(prob: Problem, nsteps: int = 1):
+def stepforward_(prob: Problem, nsteps=1, nsteps2=None):
     """stepforward!(prob[, nsteps]): advance, then clock.t += dt and clock.step += 1 per step.
+    stepforward!(prob, diags, nsteps): after each step, increment!(diags).
 
     A time-varying user flow is evaluated on the host at clock.t before each
     step (it is frozen within the step, as calcN! at S:701, S:718, S:737).
     """
     from . import traceradvectiondiffusion as tad
 
+    if nsteps2 is not None:
+        from .output import stepforward_diags_
+        return stepforward_diags_(prob, nsteps, nsteps2)
     nsteps = int(nsteps)
     if nsteps < 0:
         raise ValueError("nsteps must be >= 0")
