@@ -35,7 +35,8 @@ enum {
 };
 
 enum { PTF_FP64 = 0, PTF_MIXED = 1 };            /* arithmetic of the bulk fields   */
-enum { PTF_RK4 = 0, PTF_ETDRK4 = 1 };             /* stepper="RK4" / "ETDRK4"        */
+enum { PTF_RK4 = 0, PTF_ETDRK4 = 1,                 /* stepper="RK4" / "ETDRK4"        */
+       PTF_FILTERED_RK4 = 2, PTF_FILTERED_ETDRK4 = 3 }; /* "FilteredRK4" / "FilteredETDRK4" */
 enum {
   PTF_FLOW_STEADY = 0,    /* u,v,w sampled once on the grid (S:426-452)              */
   PTF_FLOW_PER_STEP = 1,  /* u,v,w re-uploaded by the caller at every step boundary, */
@@ -55,10 +56,18 @@ typedef struct ptf_desc {
   double iota;           /* z-diffusivity iota (reference default iota = kappa)      */
   double kappah;         /* hyperdiffusivity kappa_h                                 */
   int32_t nkappah;       /* hyperdiffusivity order n_kappa_h (Krsq^n)                */
-  int32_t stepper;       /* PTF_RK4 or PTF_ETDRK4                                    */
+  int32_t stepper;       /* PTF_RK4, PTF_ETDRK4, PTF_FILTERED_RK4, PTF_FILTERED_ETDRK4 */
   double dt;             /* time step                                                */
   int32_t flow_kind;     /* PTF_FLOW_*                                               */
-  int32_t reserved;
+  /* Filtered steppers ([FF] makefilter(grid; order, innerK, outerK, tol)):
+   * filter(K) = exp(-decay (K - innerK)^order) for K >= innerK, else 1, with
+   * K = |(kr dx, l dy, m dz)| / pi and decay = -log(tol) / (outerK - innerK)^order;
+   * sol is multiplied by it after every step's update.  filter_order = 0 selects
+   * the defaults order 4, innerK 0.65, outerK 1, tol 1e-15. */
+  int32_t filter_order;
+  double filter_innerK;
+  double filter_outerK;
+  double filter_tol;
 } ptf_desc;
 
 typedef struct ptf_problem ptf_problem;
@@ -90,6 +99,12 @@ int ptf_rfft(ptf_plan* plan, const void* in_real, void* out_complex);
  * superset of FFTW's destroy-input contract); only the real parts of the
  * kx = 0 and kx = nx/2 entries are used after the y/z inverse passes.        */
 int ptf_irfft(ptf_plan* plan, const void* in_complex, void* out_real);
+/* dealias!(fh, grid) ([FF], used by the MQG-coupled path S:225-250): zero the
+ * aliased wavenumbers of a device half spectrum of the plan's grid, kr indices
+ * from floor((1-f)/2 nx) (0-based) to the end, l and m indices in
+ * [floor((1-f)/2 n), ceil((1+f)/2 n)), f = aliased_fraction (1/3: the 2/3 rule);
+ * f <= 0 leaves fh unchanged.  Asynchronous on the plan's stream. */
+int ptf_dealias(ptf_plan* plan, void* spec, double aliased_fraction);
 
 /* ---- problems: TracerAdvectionDiffusion.Problem (S:143-216) + the
  *      [FF] FourierFlows.Problem/Clock/RK4TimeStepper/ETDRK4TimeStepper ---- */

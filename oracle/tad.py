@@ -29,6 +29,11 @@ in the reference; its published algorithm is restated where marked [FF]):
 * ``Problem.step``    -- [FF] ``RK4TimeStepper``/``ETDRK4TimeStepper`` ``stepforward!``:
                          stage algebra, RHS sum order and clock update after the step.
 * ``etd_coefficients``-- [FF] ``getetdcoeffs(dt, L; ncirc=32, rcirc=1)`` and ``getexpLs``.
+* ``makefilter``      -- [FF] ``makefilter(grid; order=4, innerK=0.65, outerK=1, tol=1e-15)``
+                         for ``FilteredRK4``/``FilteredETDRK4`` (the MQG-coupled problem's
+                         stepper, S:228): sol = filter * (update) after every step.
+* ``aliased_ranges``/``dealias`` -- [FF] ``getaliasedwavenumbers`` and ``dealias!``
+                         (aliased_fraction = 1/3: the 2/3 rule).
 * ``set_c``/``updatevars`` -- S:844-852, S:815-821.
 
 Pinning: tests/test_oracle_pins.py checks every function above against closed
@@ -234,6 +239,54 @@ def etd_coefficients(dt: float, L: np.ndarray, ncirc: int = 32, rcirc: float = 1
 
 
 # ----------------------------------------------------------------------------
+# filter and dealiasing  ([FF] utils.jl makefilter; domains.jl getaliasedwavenumbers)
+# ----------------------------------------------------------------------------
+def makefilter(grid: Grid, order=4, innerK=0.65, outerK=1.0, tol=1e-15) -> np.ndarray:
+    """filter(K) = exp(-decay (K - innerK)^order), decay = -log(tol)/(outerK - innerK)^order,
+    set to 1 where K < innerK; K = sqrt((kr dx/pi)^2 + (l dy/pi)^2 + (m dz/pi)^2) on the
+    half spectrum (realvars)."""
+    K2 = (grid.kr * (grid.Lx / grid.nx) / np.pi) ** 2
+    if grid.ndim >= 2:
+        K2 = K2 + (grid.l * (grid.Ly / grid.ny) / np.pi) ** 2
+    if grid.ndim == 3:
+        K2 = K2 + (grid.m * (grid.Lz / grid.nz) / np.pi) ** 2
+    K = np.broadcast_to(np.sqrt(K2), grid.spec_shape)
+    decay = -math.log(tol) / (outerK - innerK) ** order
+    filt = np.exp(-decay * (K - innerK) ** order)
+    return np.where(K < innerK, 1.0, filt)
+
+
+def aliased_ranges(nk: int, nkr: int, aliased_fraction: float = 1 / 3):
+    """0-based index ranges [lo, hi) of the aliased wavenumbers of a full (fftfreq) axis
+    of length nk and of the half (rfftfreq) axis of length nkr:
+    iL = floor((1 - f)/2 nk) + 1, iR = ceil((1 + f)/2 nk) as 1-based indices."""
+    if not aliased_fraction < 1:
+        raise ValueError("aliased_fraction must be less than 1")
+    if aliased_fraction <= 0:
+        return (nk // 2, nk // 2 + 1), (nkr - 1, nkr)
+    iL = math.floor((1 - aliased_fraction) / 2 * nk) + 1
+    iR = math.ceil((1 + aliased_fraction) / 2 * nk)
+    return (iL - 1, iR), (iL - 1, nkr)
+
+
+def dealias(grid: Grid, fh: np.ndarray, aliased_fraction: float = 1 / 3) -> np.ndarray:
+    """dealias!(fh, grid) on a half-spectrum array (a copy is returned): zero the aliased
+    kr columns and the aliased l (and m) rows."""
+    out = np.array(fh, copy=True)
+    if aliased_fraction <= 0:
+        return out
+    _, (rlo, rhi) = aliased_ranges(grid.nx, grid.nkr, aliased_fraction)
+    out[..., rlo:rhi] = 0
+    if grid.ndim >= 2:
+        (lo, hi), _ = aliased_ranges(grid.ny, grid.ny // 2 + 1, aliased_fraction)
+        out[..., lo:hi, :] = 0
+    if grid.ndim == 3:
+        (lo, hi), _ = aliased_ranges(grid.nz, grid.nz // 2 + 1, aliased_fraction)
+        out[lo:hi, :, :] = 0
+    return out
+
+
+# ----------------------------------------------------------------------------
 # Problem / Clock / steppers  ([FF] problem.jl, timesteppers.jl; TAD S:143-216)
 # ----------------------------------------------------------------------------
 class Clock:
@@ -253,13 +306,18 @@ class Problem:
     """
 
     def __init__(self, grid: Grid, flow, kappa=0.1, eta=None, iota=None, kappah=0.0, nkappah=0,
-                 dt=0.01, stepper="RK4"):
+                 dt=0.01, stepper="RK4", **filterkw):
         self.grid = grid
         self.flow = flow
         self.steady = not callable(flow)
         self.L = linear_operator(grid, kappa, eta, iota, kappah, nkappah)
         self.clock = Clock(dt)
-        self.stepper = stepper.upper()
+        st = stepper.upper()
+        self.filter = None
+        if st.startswith("FILTERED"):
+            self.filter = makefilter(grid, **filterkw)
+            st = st[len("FILTERED"):]
+        self.stepper = st
         self.sol = np.zeros(grid.spec_shape, dtype=np.complex128)
         self.c = np.zeros(grid.shape)
         if self.stepper == "ETDRK4":
@@ -292,6 +350,8 @@ class Problem:
                 self._rk4()
             else:
                 self._etdrk4()
+            if self.filter is not None:   # [FF] Filtered*update!: sol = filter * update
+                self.sol = self.filter * self.sol
             self.clock.t += self.clock.dt
             self.clock.step += 1
 

@@ -270,7 +270,10 @@ struct Group {
   int lb = 0;  // the last axis is stored in bricks of 2^lb planes (2D: y, 3D: z)
   bool dbl = true;
   size_t esz = 8;
-  int stepper = PTF_RK4;
+  int stepper = PTF_RK4;          // PTF_RK4 or PTF_ETDRK4 (the unfiltered form)
+  bool filtered = false;          // FilteredRK4 / FilteredETDRK4
+  int f_order = 4;
+  double f_innerK = 0.65, f_decay = 0.0;
   int flow_kind = PTF_FLOW_STEADY;
   long long nS = 0, nR = 0;  // elements per member
   double dt = 0.01;
@@ -319,8 +322,15 @@ static int validate_desc(const ptf_desc* d) {
   if (d->ndim < 1 || d->ndim > 3) return fail(PTF_ERR_INVALID, "ndim must be 1, 2 or 3");
   if (d->precision != PTF_FP64 && d->precision != PTF_MIXED)
     return fail(PTF_ERR_INVALID, "precision must be PTF_FP64 or PTF_MIXED");
-  if (d->stepper != PTF_RK4 && d->stepper != PTF_ETDRK4)
-    return fail(PTF_ERR_INVALID, "stepper must be PTF_RK4 or PTF_ETDRK4");
+  if (d->stepper < PTF_RK4 || d->stepper > PTF_FILTERED_ETDRK4)
+    return fail(PTF_ERR_INVALID, "stepper must be PTF_RK4, PTF_ETDRK4 or a filtered form");
+  if ((d->stepper == PTF_FILTERED_RK4 || d->stepper == PTF_FILTERED_ETDRK4) && d->filter_order) {
+    if (d->filter_order < 1) return fail(PTF_ERR_INVALID, "filter_order must be >= 1");
+    if (!(d->filter_outerK > d->filter_innerK))
+      return fail(PTF_ERR_INVALID, "filter_outerK must exceed filter_innerK");
+    if (!(d->filter_tol > 0 && d->filter_tol < 1))
+      return fail(PTF_ERR_INVALID, "filter_tol must be in (0, 1)");
+  }
   if (d->flow_kind < PTF_FLOW_STEADY || d->flow_kind > PTF_FLOW_SPECTRAL)
     return fail(PTF_ERR_INVALID, "unknown flow_kind");
   if (d->flow_kind == PTF_FLOW_SPECTRAL && d->ndim != 2)
@@ -445,6 +455,10 @@ static StageArgs<T> stage_args(const Group& g, int stage) {
   st.nkappah = g.nkappah;
   st.g = grid_info<T>(g);
   st.mstride = g.nS;
+  st.filtered = g.filtered && stage == 3 ? 1 : 0;
+  st.f_order = g.f_order;
+  st.f_innerK = g.f_innerK;
+  st.f_decay = g.f_decay;
   return st;
 }
 
@@ -760,7 +774,20 @@ static int group_init_once(Group& g, const ptf_desc* descs, int count, int lb) {
   g.B = count;
   g.dbl = d.precision == PTF_FP64;
   g.esz = g.dbl ? 8 : 4;
-  g.stepper = d.stepper;
+  g.filtered = d.stepper == PTF_FILTERED_RK4 || d.stepper == PTF_FILTERED_ETDRK4;
+  g.stepper = (d.stepper == PTF_ETDRK4 || d.stepper == PTF_FILTERED_ETDRK4) ? PTF_ETDRK4 : PTF_RK4;
+  if (g.filtered) {
+    const bool dflt = d.filter_order == 0;
+    const int order = dflt ? 4 : d.filter_order;
+    const double innerK = dflt ? 0.65 : d.filter_innerK;
+    const double outerK = dflt ? 1.0 : d.filter_outerK;
+    const double tol = dflt ? 1e-15 : d.filter_tol;
+    double span = 1.0;
+    for (int i = 0; i < order; ++i) span *= (outerK - innerK);
+    g.f_order = order;
+    g.f_innerK = innerK;
+    g.f_decay = -std::log(tol) / span;
+  }
   g.flow_kind = d.flow_kind;
   g.nS = (long long)g.nxh * g.ny * g.nz;
   g.nR = (long long)g.nx * g.ny * g.nz;
@@ -1263,6 +1290,38 @@ int ptf_irfft(ptf_plan* plan, const void* in_complex, void* out_real) {
   return PTF_OK;
 }
 
+// [FF] getaliasedwavenumbers(nk, nkr, f): 1-based iL = floor((1-f)/2 nk) + 1,
+// iR = ceil((1+f)/2 nk); full axis aliased indices [iL-1, iR), half axis [iL-1, nkr)
+static void aliased_range(int nk, double f, int* lo, int* hi) {
+  const int iL = (int)std::floor((1.0 - f) / 2.0 * nk) + 1;
+  const int iR = (int)std::ceil((1.0 + f) / 2.0 * nk);
+  *lo = iL - 1;
+  *hi = iR;
+}
+
+int ptf_dealias(ptf_plan* plan, void* spec, double aliased_fraction) {
+  if (!plan || !spec) return fail(PTF_ERR_INVALID, "NULL argument");
+  if (!(aliased_fraction < 1.0)) return fail(PTF_ERR_INVALID, "aliased_fraction must be < 1");
+  if (!(aliased_fraction > 0.0)) return PTF_OK;   // [FF] dealias! does nothing then
+  Group& g = plan->g;
+  int rlo, rhi, ylo = 0, yhi = 0, zlo = 0, zhi = 0;
+  aliased_range(g.nx, aliased_fraction, &rlo, &rhi);
+  rhi = g.nxh;
+  if (g.ndim >= 2) aliased_range(g.ny, aliased_fraction, &ylo, &yhi);
+  if (g.ndim == 3) aliased_range(g.nz, aliased_fraction, &zlo, &zhi);
+  const long long n = g.nS;
+  const int th = 256;
+  const unsigned blocks = (unsigned)std::min<long long>((n + th - 1) / th, 65536);
+  if (g.dbl)
+    hipLaunchKernelGGL(dealias_kernel<double>, dim3(blocks), dim3(th), 0, plan->stream,
+                       (cpx<double>*)spec, n, g.nxh, g.ny, rlo, rhi, ylo, yhi, zlo, zhi);
+  else
+    hipLaunchKernelGGL(dealias_kernel<float>, dim3(blocks), dim3(th), 0, plan->stream,
+                       (cpx<float>*)spec, n, g.nxh, g.ny, rlo, rhi, ylo, yhi, zlo, zhi);
+  HIPCHK(hipGetLastError());
+  return PTF_OK;
+}
+
 // ----- problems ----------------------------------------------------------------
 int ptf_problem_create(const ptf_desc* desc, ptf_problem** out) {
   if (!out) return fail(PTF_ERR_INVALID, "out is NULL");
@@ -1700,6 +1759,8 @@ int ptf_ensemble_create(const ptf_desc* descs, int32_t count, ptf_ensemble** out
     CHK(validate_desc(&descs[i]));
     if (descs[i].flow_kind == PTF_FLOW_SPECTRAL)
       return fail(PTF_ERR_UNSUPPORTED, "ensemble members take steady or per-step flows");
+    if (descs[i].stepper == PTF_FILTERED_RK4 || descs[i].stepper == PTF_FILTERED_ETDRK4)
+      return fail(PTF_ERR_UNSUPPORTED, "ensemble members take the RK4 or ETDRK4 stepper");
   }
   std::unique_ptr<ptf_ensemble> e(new ptf_ensemble());
   e->where.resize(count);

@@ -288,7 +288,32 @@ struct StageArgs {
   int nkappah;
   GridInfo g;
   long long mstride;    // spectral elements per member
+  // filtered steppers: sol *= filter(K) after the last stage's update
+  int filtered;
+  int f_order;
+  double f_innerK;
+  double f_decay;
 };
+
+// [FF] makefilter(grid; order, innerK, outerK, tol) at one spectral element:
+// K = sqrt((kr dx/pi)^2 + (l dy/pi)^2 + (m dz/pi)^2), where k dx/pi = 2 i/n for the
+// signed wavenumber index i; filter = 1 for K < innerK, else exp(-decay (K - innerK)^order).
+__device__ __forceinline__ double filter_value(const GridInfo& g, int order, double innerK,
+                                               double decay, int ikx, int iky, int ikz) {
+  const double a = 2.0 * ikx / g.nx;
+  double K2 = a * a;
+  if (g.ndim >= 2) {
+    const double b = 2.0 * fftfreq_index(iky, g.ny) / g.ny;
+    K2 += b * b;
+  }
+  if (g.ndim == 3) {
+    const double c = 2.0 * fftfreq_index(ikz, g.nz) / g.nz;
+    K2 += c * c;
+  }
+  const double K = sqrt(K2);
+  if (K < innerK) return 1.0;
+  return exp(-decay * ipow_d(K - innerK, order));
+}
 
 // (hi, lo) + y -> (hi', lo') in working precision: TwoSum of hi and y, the error
 // and lo folded together, then a FastTwoSum renormalisation (Knuth; Dekker).
@@ -379,7 +404,11 @@ __device__ __forceinline__ cpx<T> rk4_finish(const StageArgs<T>& a, long long ub
   } else {
     if (!a.sol_lo) {
       // o.s = sol + dt (RHS_1/6 + RHS_2/3 + RHS_3/3), stored by stage 3
-      const cpx<T> ns = {o.s.re + dt * (rhs.re / (T)6), o.s.im + dt * (rhs.im / (T)6)};
+      cpx<T> ns = {o.s.re + dt * (rhs.re / (T)6), o.s.im + dt * (rhs.im / (T)6)};
+      if (a.filtered) {
+        const T f = (T)filter_value(a.g, a.f_order, a.f_innerK, a.f_decay, ikx, iky, ikz);
+        ns = {f * ns.re, f * ns.im};
+      }
       (a.sol + ubase)[off] = ns;
       return ns;
     }
@@ -388,6 +417,15 @@ __device__ __forceinline__ cpx<T> rk4_finish(const StageArgs<T>& a, long long ub
     T hr, lr, hi_, li_;
     comp_add(o.s.re, o.lo.re, dt * tot.re, hr, lr);
     comp_add(o.s.im, o.lo.im, dt * tot.im, hi_, li_);
+    if (a.filtered) {
+      // (hi + lo) f in FP64, split again into a working-precision pair
+      const double f = filter_value(a.g, a.f_order, a.f_innerK, a.f_decay, ikx, iky, ikz);
+      const double xr = ((double)hr + (double)lr) * f, xi = ((double)hi_ + (double)li_) * f;
+      hr = (T)xr;
+      hi_ = (T)xi;
+      lr = (T)(xr - (double)hr);
+      li_ = (T)(xi - (double)hi_);
+    }
     (a.sol + ubase)[off] = {hr, hi_};
     (a.sol_lo + ubase)[off] = {lr, li_};
     return {hr, hi_};
@@ -396,7 +434,7 @@ __device__ __forceinline__ cpx<T> rk4_finish(const StageArgs<T>& a, long long ub
 
 template <typename T>
 __device__ __forceinline__ cpx<T> etdrk4_update(const StageArgs<T>& a, long long ubase,
-                                                uint32_t off, cpx<T> N) {
+                                                uint32_t off, int ikx, int iky, int ikz, cpx<T> N) {
   // coefficients are FP64 arrays; the update runs in FP64 and rounds once
   const double nre = (double)N.re, nim = (double)N.im;
   auto sol_at = [&](double& re, double& im) {
@@ -449,7 +487,12 @@ __device__ __forceinline__ cpx<T> etdrk4_update(const StageArgs<T>& a, long long
       ai += (double)al.im;
     }
     const double g = (a.gam + ubase)[off];
-    const double sr = ar + g * nre, si = ai + g * nim;
+    double sr = ar + g * nre, si = ai + g * nim;
+    if (a.filtered) {
+      const double f = filter_value(a.g, a.f_order, a.f_innerK, a.f_decay, ikx, iky, ikz);
+      sr *= f;
+      si *= f;
+    }
     const cpx<T> hi = {(T)sr, (T)si};
     (a.sol + ubase)[off] = hi;
     if (a.sol_lo) (a.sol_lo + ubase)[off] = {(T)(sr - (double)hi.re), (T)(si - (double)hi.im)};
@@ -467,7 +510,7 @@ __device__ __forceinline__ cpx<T> stage_epilogue(const StageArgs<T>& a, long lon
     const Rk4Ops<T> o = rk4_load(a, ubase, off);
     return rk4_finish(a, ubase, off, member, ikx, iky, ikz, N, o);
   }
-  return etdrk4_update(a, ubase, off, N);
+  return etdrk4_update(a, ubase, off, ikx, iky, ikz, N);
 }
 
 }  // namespace ptf

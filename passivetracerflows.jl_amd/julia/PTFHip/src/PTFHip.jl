@@ -28,7 +28,9 @@ const libptf = get(ENV, "PTF_LIB", joinpath(@__DIR__, "..", "..", "..", "lib", "
 
 # ---- C ABI mirror (include/ptf.h) ------------------------------------------------
 const PTF_FP64, PTF_MIXED = Int32(0), Int32(1)
-const PTF_RK4, PTF_ETDRK4 = Int32(0), Int32(1)
+const PTF_RK4, PTF_ETDRK4, PTF_FILTERED_RK4, PTF_FILTERED_ETDRK4 = Int32.((0, 1, 2, 3))
+const STEPPERS = Dict("RK4" => PTF_RK4, "ETDRK4" => PTF_ETDRK4, "FilteredRK4" => PTF_FILTERED_RK4,
+                      "FilteredETDRK4" => PTF_FILTERED_ETDRK4)
 const PTF_FLOW_STEADY, PTF_FLOW_PER_STEP, PTF_FLOW_SPECTRAL = Int32(0), Int32(1), Int32(2)
 
 struct PtfDesc
@@ -44,7 +46,10 @@ struct PtfDesc
   stepper::Int32
   dt::Float64
   flow_kind::Int32
-  reserved::Int32
+  filter_order::Int32          # 0: FourierFlows' makefilter defaults
+  filter_innerK::Float64
+  filter_outerK::Float64
+  filter_tol::Float64
 end
 
 lasterror() = unsafe_string(ccall((:ptf_last_error, libptf), Cstring, ()))
@@ -149,8 +154,8 @@ function Problem(dev::ROCm, flow; nx=128, Lx=2π, ny=nx, Ly=Lx, nz=nx, Lz=Lx, κ
   n = (nx, nd >= 2 ? ny : 1, nd >= 3 ? nz : 1)
   L = (Float64(Lx), nd >= 2 ? Float64(Ly) : 1.0, nd >= 3 ? Float64(Lz) : 1.0)
   d = PtfDesc(nd, T == Float64 ? PTF_FP64 : PTF_MIXED, Int64.(n), L, κ, η, ι, κh, nκh,
-              stepper == "RK4" ? PTF_RK4 : PTF_ETDRK4, dt,
-              flow.steadyflow ? PTF_FLOW_STEADY : PTF_FLOW_PER_STEP, 0)
+              STEPPERS[stepper], dt,
+              flow.steadyflow ? PTF_FLOW_STEADY : PTF_FLOW_PER_STEP, 0, 0.0, 0.0, 0.0)
   h = Ref{Ptr{Cvoid}}(C_NULL)
   check(ccall((:ptf_problem_create, libptf), Cint, (Ref{PtfDesc}, Ref{Ptr{Cvoid}}), d, h))
   vars = (c = zeros(Float64, n[1:nd]...), ch = zeros(ComplexF64, n[1] ÷ 2 + 1, n[2:nd]...))

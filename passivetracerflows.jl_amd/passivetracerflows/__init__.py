@@ -10,8 +10,8 @@ from . import fourierflows as FourierFlows
 from . import traceradvectiondiffusion as TracerAdvectionDiffusion
 from .ensemble import Ensemble
 from .output import Diagnostic, Output, increment_, jldopen, record_, save_records, saveoutput, saveproblem
-from .fourierflows import (GPU, Clock, DeviceArray, OneDGrid, ROCm, ThreeDGrid, TwoDGrid, gridpoints,
-                           ldiv_, mul_, step_until_, stepforward_)
+from .fourierflows import (GPU, Clock, DeviceArray, OneDGrid, ROCm, ThreeDGrid, TwoDGrid, dealias_,
+                           gridpoints, ldiv_, mul_, step_until_, stepforward_)
 from .traceradvectiondiffusion import (OneDAdvectingFlow, SpectralStreamfunctionFlow, ThreeDAdvectingFlow,
                                        TwoDAdvectingFlow, set_c_, updatevars_)
 
@@ -19,7 +19,7 @@ stepforward = stepforward_
 
 __all__ = [
     "FourierFlows", "TracerAdvectionDiffusion", "Ensemble", "ROCm", "GPU", "Clock", "DeviceArray",
-    "OneDGrid", "TwoDGrid", "ThreeDGrid", "gridpoints", "mul_", "ldiv_", "stepforward_", "step_until_",
+    "OneDGrid", "TwoDGrid", "ThreeDGrid", "gridpoints", "mul_", "ldiv_", "dealias_", "stepforward_", "step_until_",
     "OneDAdvectingFlow", "TwoDAdvectingFlow", "ThreeDAdvectingFlow", "SpectralStreamfunctionFlow",
     "set_c_", "updatevars_", "Diagnostic", "increment_", "Output", "saveproblem", "saveoutput",
     "save_records", "record_", "jldopen",

@@ -14,7 +14,9 @@ LIB_PATH = os.environ.get("PTF_LIB", os.path.join(os.path.dirname(_HERE), "lib",
 
 PTF_OK, PTF_ERR_INVALID, PTF_ERR_DEVICE, PTF_ERR_NOMEM, PTF_ERR_UNSUPPORTED = 0, 1, 2, 3, 4
 PTF_FP64, PTF_MIXED = 0, 1
-PTF_RK4, PTF_ETDRK4 = 0, 1
+PTF_RK4, PTF_ETDRK4, PTF_FILTERED_RK4, PTF_FILTERED_ETDRK4 = 0, 1, 2, 3
+STEPPERS = {"RK4": PTF_RK4, "ETDRK4": PTF_ETDRK4, "FILTEREDRK4": PTF_FILTERED_RK4,
+            "FILTEREDETDRK4": PTF_FILTERED_ETDRK4}
 PTF_FLOW_STEADY, PTF_FLOW_PER_STEP, PTF_FLOW_SPECTRAL = 0, 1, 2
 
 
@@ -44,7 +46,10 @@ class ptf_desc(C.Structure):
         ("stepper", C.c_int32),
         ("dt", C.c_double),
         ("flow_kind", C.c_int32),
-        ("reserved", C.c_int32),
+        ("filter_order", C.c_int32),
+        ("filter_innerK", C.c_double),
+        ("filter_outerK", C.c_double),
+        ("filter_tol", C.c_double),
     ]
 
 
@@ -89,6 +94,7 @@ SIGNATURES = {
     "ptf_problem_slot_name": [_VP, _I32, C.c_char_p, _I32],
     "ptf_problem_time_slot": [_VP, _I32, _I32, _DP],
     "ptf_step_profiled": [_VP, _I64, _DP],
+    "ptf_dealias": [_VP, _VP, C.c_double],
     "ptf_step_record": [_VP, _I64, _I64, _I64, C.POINTER(_I64), _DP, C.POINTER(_I64), _DP, _DP,
                         _DP],
     "ptf_problem_set_variant": [_VP, _I32, _I32, _I32, _I32],

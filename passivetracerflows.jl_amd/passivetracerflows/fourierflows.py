@@ -53,7 +53,7 @@ def _kscale(n, L):
 class _Grid:
     ndim = 0
 
-    def __init__(self, dev, nx, Lx, ny, Ly, nz, Lz, T):
+    def __init__(self, dev, nx, Lx, ny, Ly, nz, Lz, T, aliased_fraction=1 / 3):
         if not isinstance(dev, ROCm):
             raise TypeError("this grid lives on the ROCm device")
         self.device = dev
@@ -93,6 +93,7 @@ class _Grid:
             self.Krsq = self.kr ** 2 + self.l ** 2 + self.m ** 2
             self.shape = (self.nz, self.ny, self.nx)
         self.spec_shape = self.shape[:-1] + (self.nkr,)
+        self.aliased_fraction = float(aliased_fraction)
         self._plan = None
 
     @property
@@ -112,23 +113,26 @@ class _Grid:
 class OneDGrid(_Grid):
     ndim = 1
 
-    def __init__(self, dev, nx=128, Lx=2 * np.pi, T=np.float64):
-        super().__init__(dev, nx, Lx, 1, 1.0, 1, 1.0, T)
+    def __init__(self, dev, nx=128, Lx=2 * np.pi, T=np.float64, aliased_fraction=1 / 3):
+        super().__init__(dev, nx, Lx, 1, 1.0, 1, 1.0, T, aliased_fraction)
 
 
 class TwoDGrid(_Grid):
     ndim = 2
 
-    def __init__(self, dev, nx=128, Lx=2 * np.pi, ny=None, Ly=None, T=np.float64):
-        super().__init__(dev, nx, Lx, nx if ny is None else ny, Lx if Ly is None else Ly, 1, 1.0, T)
+    def __init__(self, dev, nx=128, Lx=2 * np.pi, ny=None, Ly=None, T=np.float64,
+                 aliased_fraction=1 / 3):
+        super().__init__(dev, nx, Lx, nx if ny is None else ny, Lx if Ly is None else Ly, 1, 1.0, T,
+                         aliased_fraction)
 
 
 class ThreeDGrid(_Grid):
     ndim = 3
 
-    def __init__(self, dev, nx=128, Lx=2 * np.pi, ny=None, Ly=None, nz=None, Lz=None, T=np.float64):
+    def __init__(self, dev, nx=128, Lx=2 * np.pi, ny=None, Ly=None, nz=None, Lz=None, T=np.float64,
+                 aliased_fraction=1 / 3):
         super().__init__(dev, nx, Lx, nx if ny is None else ny, Lx if Ly is None else Ly,
-                         nx if nz is None else nz, Lx if Lz is None else Lz, T)
+                         nx if nz is None else nz, Lx if Lz is None else Lz, T, aliased_fraction)
 
 
 def gridpoints(grid):
@@ -222,6 +226,14 @@ def ldiv_(out, plan: RFFTPlan, a):
     return out
 
 
+def dealias_(fh, grid):
+    """dealias!(fh, grid): zero the aliased wavenumbers of the half spectrum fh (a device
+    array; grid.aliased_fraction, default 1/3 = the 2/3 rule)."""
+    call("ptf_dealias", grid.rfftplan._h, _devptr(fh), float(grid.aliased_fraction))
+    call("ptf_device_synchronize")
+    return None
+
+
 # ---------------------------------------------------------------------------
 # Problem / Clock / stepforward!
 # ---------------------------------------------------------------------------
@@ -255,7 +267,7 @@ class Problem:
     (complex128, spectral shape) and assigning it uploads.
     """
 
-    def __init__(self, eqn, stepper, dt, grid, vars, params):
+    def __init__(self, eqn, stepper, dt, grid, vars, params, **stepperkwargs):
         from . import traceradvectiondiffusion as tad  # the only physics module
 
         self.eqn = eqn
@@ -264,12 +276,21 @@ class Problem:
         self.params = params
         self.stepper = stepper
         st = stepper.upper()
-        if st not in ("RK4", "ETDRK4"):
+        if st not in _lib.STEPPERS:
             raise ValueError(f"stepper {stepper!r} is not implemented on the ROCm device "
-                             "(RK4 and ETDRK4 are)")
+                             "(RK4, ETDRK4, FilteredRK4 and FilteredETDRK4 are)")
+        filterkw = None
+        if st.startswith("FILTERED"):
+            unknown = set(stepperkwargs) - {"order", "innerK", "outerK", "tol"}
+            if unknown:
+                raise TypeError(f"unknown filter keywords {sorted(unknown)}")
+            filterkw = dict(order=4, innerK=0.65, outerK=1.0, tol=1e-15)
+            filterkw.update(stepperkwargs)
+        elif stepperkwargs:
+            raise TypeError(f"stepper {stepper!r} takes no keywords")
         self.clock = Clock(float(dt))
         self.timestepper = st
-        self._h = tad._create_handle(self, st, float(dt))
+        self._h = tad._create_handle(self, st, float(dt), filterkw)
 
     # prob.sol
     @property

@@ -330,7 +330,7 @@ def _flow_kind(params):
     return _lib.PTF_FLOW_STEADY if params.steady else _lib.PTF_FLOW_PER_STEP
 
 
-def _desc(params, grid, stepper, dt):
+def _desc(params, grid, stepper, dt, filterkw=None):
     d = _lib.ptf_desc()
     d.ndim = grid.ndim
     d.precision = grid.precision
@@ -342,9 +342,15 @@ def _desc(params, grid, stepper, dt):
     d.iota = params.ι
     d.kappah = params.κh
     d.nkappah = params.nκh
-    d.stepper = _lib.PTF_RK4 if stepper == "RK4" else _lib.PTF_ETDRK4
+    d.stepper = _lib.STEPPERS[stepper.upper()]
     d.dt = dt
     d.flow_kind = _flow_kind(params)
+    if filterkw:
+        # [FF] makefilter keywords (defaults order=4, innerK=0.65, outerK=1, tol=1e-15)
+        d.filter_order = int(filterkw.get("order", 4))
+        d.filter_innerK = float(filterkw.get("innerK", 0.65))
+        d.filter_outerK = float(filterkw.get("outerK", 1.0))
+        d.filter_tol = float(filterkw.get("tol", 1e-15))
     return d
 
 
@@ -354,9 +360,9 @@ def _upload_fields(h, fields):
     call("ptf_set_flow", h, *ptrs)
 
 
-def _create_handle(prob, stepper, dt):
+def _create_handle(prob, stepper, dt, filterkw=None):
     params, grid = prob.params, prob.grid
-    d = _desc(params, grid, stepper, dt)
+    d = _desc(params, grid, stepper, dt, filterkw)
     h = C.c_void_p()
     call("ptf_problem_create", C.byref(d), C.byref(h))
     h = h.value

@@ -268,3 +268,97 @@ def test_reference_analytic_cases_oracle(name):
 def test_unknown_stepper():
     with pytest.raises(ValueError):
         O.Problem(O.Grid(1, 8), (np.zeros(8),), stepper="AB3")
+
+
+# ---------------------------------------------------------------------------
+# filtered steppers and dealiasing ([FF] makefilter, Filtered*update!, dealias!)
+# ---------------------------------------------------------------------------
+def test_makefilter_closed_forms():
+    """K = |(kr dx, l dy, m dz)|/pi is 2|i|/n for index i whatever L is; the filter is 1
+    below innerK and equals tol at K = outerK (the x-Nyquist column for outerK = 1)."""
+    g = O.Grid(2, 64)
+    f = O.makefilter(g)
+    Kx = 2 * np.arange(33) / 64
+    decay = -np.log(1e-15) / 0.35 ** 4
+    expect = np.where(Kx < 0.65, 1.0, np.exp(-decay * (Kx - 0.65) ** 4))
+    assert np.allclose(f[0], expect, rtol=1e-13, atol=0)
+    assert abs(f[0, 32] - 1e-15) <= 1e-27
+    assert np.all(f[0, Kx < 0.65] == 1.0)
+    assert np.all(np.diff(f[0]) <= 0)
+    # independent of the domain lengths; even in l
+    g2 = O.Grid(2, 64, Lx=3.0, Ly=5.0)
+    assert np.allclose(O.makefilter(g2), f, rtol=1e-11, atol=1e-300)
+    ky = np.fft.fftfreq(64, 1 / 64)
+    for i in range(1, 32):
+        j = int(np.where(ky == -ky[i])[0][0])
+        assert np.allclose(f[i], f[j], rtol=1e-15, atol=0)
+    # 3D: K adds the m direction; custom order/innerK/outerK/tol reach tol at outerK
+    g3 = O.Grid(3, 16)
+    f3 = O.makefilter(g3, order=6, innerK=0.5, outerK=0.75, tol=1e-10)
+    # index 6 on every axis: K = sqrt(3) * 0.75 > outerK; index (0, 0, 6): K = 0.75
+    assert abs(f3[0, 0, 6] - 1e-10) <= 1e-22
+    assert f3[6, 6, 6] < 1e-10
+
+
+def _zero_flow_problem(ndim, n, stepper, kappa, dt, seed, **fk):
+    g = O.Grid(ndim, n)
+    rng = np.random.default_rng(seed)
+    flows = tuple(np.zeros(g.shape) for _ in range(ndim))
+    p = O.Problem(g, flows, kappa=kappa, dt=dt, stepper=stepper, **fk)
+    p.set_c(rng.standard_normal(g.shape))
+    return g, p
+
+
+@pytest.mark.parametrize("ndim,n", [(1, 64), (2, 32), (3, 16)])
+def test_filtered_rk4_zero_flow_is_filter_times_rk4_polynomial(ndim, n):
+    """u = 0: each FilteredRK4 step multiplies sol by filter * R(dt L), R the RK4 polynomial
+    (the filter acts once per step, after the update)."""
+    g, p = _zero_flow_problem(ndim, n, "FilteredRK4", 0.02, 0.1, 3)
+    sol0 = p.sol.copy()
+    p.stepforward(7)
+    z = 0.1 * p.L
+    R = 1 + z + z ** 2 / 2 + z ** 3 / 6 + z ** 4 / 24
+    expect = (O.makefilter(g) * R) ** 7 * sol0
+    assert np.linalg.norm(p.sol - expect) <= 1e-13 * np.linalg.norm(expect)
+
+
+def test_filtered_etdrk4_zero_flow_is_filter_times_exp():
+    g, p = _zero_flow_problem(2, 32, "FilteredETDRK4", 0.02, 0.1, 4, order=6, innerK=0.5,
+                              outerK=0.9, tol=1e-12)
+    sol0 = p.sol.copy()
+    p.stepforward(5)
+    expect = (O.makefilter(g, order=6, innerK=0.5, outerK=0.9, tol=1e-12)
+              * np.exp(0.1 * p.L)) ** 5 * sol0
+    assert np.linalg.norm(p.sol - expect) <= 1e-13 * np.linalg.norm(expect)
+
+
+@pytest.mark.parametrize("nk", [8, 16, 64, 128, 256, 1024])
+def test_aliased_ranges_two_thirds_rule(nk):
+    """The aliased set of a full axis is one contiguous block around the Nyquist index with
+    all kept |k| < nk/3 and about a third of the modes; the half axis drops the same kr."""
+    (lo, hi), (rlo, rhi) = O.aliased_ranges(nk, nk // 2 + 1)
+    k = np.fft.fftfreq(nk, 1 / nk)
+    al = np.zeros(nk, bool)
+    al[lo:hi] = True
+    assert al[nk // 2]                        # Nyquist is aliased
+    assert np.all(np.abs(k[~al]) < nk / 3)
+    assert abs(al.sum() - nk / 3) <= 2
+    assert rlo == lo and rhi == nk // 2 + 1
+    # f = 0: [FF] keeps only the Nyquist entries in the ranges and dealias! does nothing
+    (zlo, zhi), (zr0, zr1) = O.aliased_ranges(nk, nk // 2 + 1, 0.0)
+    assert (zlo, zhi) == (nk // 2, nk // 2 + 1) and (zr0, zr1) == (nk // 2, nk // 2 + 1)
+
+
+def test_dealias_zeroes_exactly_the_aliased_band():
+    g = O.Grid(3, 16, ny=32, nz=8)
+    fh = np.ones(g.spec_shape, np.complex128)
+    d = O.dealias(g, fh)
+    (ylo, yhi), _ = O.aliased_ranges(32, 17)
+    (zlo, zhi), _ = O.aliased_ranges(8, 5)
+    _, (rlo, _) = O.aliased_ranges(16, 9)
+    for iz in range(8):
+        for iy in range(32):
+            for ix in range(9):
+                zero = (ix >= rlo) or (ylo <= iy < yhi) or (zlo <= iz < zhi)
+                assert d[iz, iy, ix] == (0 if zero else 1)
+    assert np.array_equal(O.dealias(g, fh, 0.0), fh)
