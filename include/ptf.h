@@ -41,10 +41,35 @@ enum {
   PTF_FLOW_STEADY = 0,    /* u,v,w sampled once on the grid (S:426-452)              */
   PTF_FLOW_PER_STEP = 1,  /* u,v,w re-uploaded by the caller at every step boundary, */
                           /* i.e. evaluated at clock.t (S:701, S:718, S:737)         */
-  PTF_FLOW_SPECTRAL = 2   /* built-in: psi_hat(t) = psi0_hat exp(-i omega t),        */
+  PTF_FLOW_SPECTRAL = 2,  /* built-in: psi_hat(t) = psi0_hat exp(-i omega t),        */
                           /* u = -d_y psi, v = d_x psi, synthesised on the GPU at     */
                           /* clock.t once per step (2D only)                         */
+  PTF_FLOW_ANALYTIC = 3   /* built-in: sums of separable terms (ptf_flow_term),     */
+                          /* evaluated on the GPU at clock.t once per step (at set   */
+                          /* time only when no term depends on t)                    */
 };
+
+/* One separable term of an analytic flow, added to velocity component
+ * `component` (0 = u, 1 = v, 2 = w):
+ *   amp * T(omega t + tphase) * X(kx x + px) * Y(ky y + py) * Z(kz z + pz),
+ * with X, Y, Z selected by fx, fy, fz and T by time_fn among
+ * PTF_FN_ONE (1), PTF_FN_SIN, PTF_FN_COS.  The factors are multiplied in the
+ * order ((amp T) X) Y) Z and the terms of a component summed in array order.
+ * Examples: the cellular flow psi = (U/k) sin kx sin ky is
+ * u = -U sin(kx) cos(ky), v = U cos(kx) sin(ky); the ABC flow is
+ * u = A sin z + C cos y, v = B sin x + A cos z, w = C sin y + B cos x. */
+enum { PTF_FN_ONE = 0, PTF_FN_SIN = 1, PTF_FN_COS = 2 };
+typedef struct ptf_flow_term {
+  int32_t component;
+  int32_t fx, fy, fz;
+  double kx, ky, kz;
+  double px, py, pz;
+  double amp;
+  int32_t time_fn;
+  int32_t reserved;
+  double omega, tphase;
+} ptf_flow_term;
+#define PTF_MAX_FLOW_TERMS 16
 
 typedef struct ptf_desc {
   int32_t ndim;          /* 1, 2 or 3                                                */
@@ -117,6 +142,12 @@ int ptf_problem_get_stream(ptf_problem* prob, void** hip_stream);
 int ptf_set_flow(ptf_problem* prob, const double* u, const double* v, const double* w);
 /* PTF_FLOW_SPECTRAL: psi0 (interleaved complex S-field) and omega (real S-field). */
 int ptf_set_spectral_flow(ptf_problem* prob, const double* psih0, const double* omega);
+/* PTF_FLOW_ANALYTIC: the flow as nterms (<= PTF_MAX_FLOW_TERMS) separable terms;
+ * the x, y, z factors are tabulated on the host in FP64 once (the grid points of
+ * the problem), the time factors at clock.t before every step.  This replaces
+ * the per-step host evaluation and upload of a time-varying user flow (the
+ * u(x, y, t) closures of S:701, S:718, S:737) for flows of this form. */
+int ptf_set_analytic_flow(ptf_problem* prob, const ptf_flow_term* terms, int32_t nterms);
 /* set_c!(prob, c) (S:844-852, S:872): sol = R2C(c); host FP64 R-field. */
 int ptf_set_c(ptf_problem* prob, const double* c);
 /* stepforward!(prob, nsteps) [FF]: clock.t += dt, clock.step += 1 after each step. */

@@ -47,6 +47,7 @@ static int fail(int code, const std::string& msg) {
 // kernel registry (ptf_registry.hip, one object per precision and kind)
 // ---------------------------------------------------------------------------
 enum {
+  K_ANALYTIC = 20,  // analytic_flow_kernel (not in the variant registry)
   K_COL_FWD = 0, K_COL_INV = 1, K_COL_D1 = 2, K_COL_D2 = 3, K_COL_STAGE = 4, K_COL_FLOWSYN = 5,
   K_COL_STAGE_D1 = 6,
   K_ROW_C2R = 10, K_ROW_R2C = 11, K_ROW_ADV1 = 12, K_ROW_ADV2 = 13, K_ROW_ADV3 = 14
@@ -86,6 +87,7 @@ static const char* kind_name(int kind) {
     case K_ROW_ADV1: return "row_advect1";
     case K_ROW_ADV2: return "row_advect2";
     case K_ROW_ADV3: return "row_advect3";
+    case K_ANALYTIC: return "analytic_flow";
   }
   return "?";
 }
@@ -114,15 +116,19 @@ struct Slot {
   std::vector<unsigned char> args;  // ColArgs<T> or RowArgs<T>
   bool writes_state = false;        // launch modifies sol (timing must restore it)
   int brick = 0;                    // log2 brick: columns along a bricked line / bricked rows
+  int fixed_threads = 0;            // > 0: a kernel outside the registry with this launch shape
+  unsigned fixed_blocks = 0;
 
   bool is_col() const { return kind < 10; }
   bool has_occ() const { return kind == 3 || kind == 6 || kind == 13 || kind == 14; }
   bool has_par() const { return kind == 13 || kind == 14; }
   int threads() const {
+    if (fixed_threads > 0) return fixed_threads;
     if (is_col()) return W * (N / R);
     return (par ? (kind == 14 ? 3 : 2) : 1) * W * ((N / 2) / R);
   }
   dim3 grid() const {
+    if (fixed_threads > 0) return dim3(fixed_blocks, 1, 1);
     if (is_col()) return dim3((unsigned)((lines + W - 1) / W), (unsigned)other, (unsigned)members);
     return dim3((unsigned)((lines + W - 1) / W), (unsigned)members, 1);
   }
@@ -288,6 +294,8 @@ struct Group {
   std::vector<Slot> prologue;      // first inverse pass of a step when w0/w1 are not primed
   std::vector<Slot> stages[4];
   std::vector<Slot> flowsyn;
+  std::vector<ptf_flow_term> aterms;   // PTF_FLOW_ANALYTIC
+  DevBuf atab;                         // x, y, z factor tables of the terms
   // w0/w1 hold the first inverse pass of the current state (the fused last
   // kernel of each step leaves them so); anything else that writes sol or the
   // transform scratch clears this
@@ -331,7 +339,7 @@ static int validate_desc(const ptf_desc* d) {
     if (!(d->filter_tol > 0 && d->filter_tol < 1))
       return fail(PTF_ERR_INVALID, "filter_tol must be in (0, 1)");
   }
-  if (d->flow_kind < PTF_FLOW_STEADY || d->flow_kind > PTF_FLOW_SPECTRAL)
+  if (d->flow_kind < PTF_FLOW_STEADY || d->flow_kind > PTF_FLOW_ANALYTIC)
     return fail(PTF_ERR_INVALID, "unknown flow_kind");
   if (d->flow_kind == PTF_FLOW_SPECTRAL && d->ndim != 2)
     return fail(PTF_ERR_UNSUPPORTED, "the built-in spectral flow is 2D only");
@@ -978,11 +986,36 @@ static int download_real(Group& g, double* dst, const void* src, size_t n, hipSt
   return PTF_OK;
 }
 
+// time factor T(omega t + phase) of an analytic flow term
+static double term_time_factor(const ptf_flow_term& k, double t) {
+  const double a = k.omega * t + k.tphase;
+  return k.time_fn == PTF_FN_SIN ? std::sin(a) : (k.time_fn == PTF_FN_COS ? std::cos(a) : 1.0);
+}
+
+template <typename T>
+static void patch_analytic(Group& g, Slot& s, double t) {
+  AnalyticArgs<T>& a = args_of<AnalyticArgs<T>>(s);
+  for (size_t k = 0; k < g.aterms.size(); ++k)
+    a.amp[k] = g.aterms[k].amp * term_time_factor(g.aterms[k], t);
+}
+
+// set the clock of the per-step flow synthesis launches to t
+static void patch_flowsyn(Group& g, double t) {
+  if (g.flowsyn.empty()) return;
+  Slot& s = g.flowsyn[0];
+  if (s.kind == K_ANALYTIC) {
+    if (g.dbl) patch_analytic<double>(g, s, t);
+    else patch_analytic<float>(g, s, t);
+  } else if (g.dbl) {
+    args_of<ColArgs<double>>(s).t = t;
+  } else {
+    args_of<ColArgs<float>>(s).t = t;
+  }
+}
+
 static int group_step_launches(Group& g, hipStream_t st, double t) {
   if (!g.flowsyn.empty()) {
-    Slot& s = g.flowsyn[0];
-    if (g.dbl) args_of<ColArgs<double>>(s).t = t;
-    else args_of<ColArgs<float>>(s).t = t;
+    patch_flowsyn(g, t);
     for (auto& fs : g.flowsyn) CHK(launch(fs, st));
   }
   if (!g.primed)
@@ -1390,6 +1423,101 @@ int ptf_set_spectral_flow(ptf_problem* prob, const double* psih0, const double* 
   return PTF_OK;
 }
 
+}  // extern "C"
+
+template <typename T>
+static int set_analytic_impl(ptf_problem* prob, const ptf_flow_term* terms, int nterms) {
+  Group& g = prob->g;
+  const int ns[3] = {g.nx, g.ny, g.nz};
+  // factor tables at the grid points x_j = -L/2 + j L/n (rounded once from long double)
+  size_t off[3];
+  size_t pos = 0;
+  for (int d = 0; d < 3; ++d) {
+    off[d] = pos;
+    pos += (size_t)nterms * (d < g.ndim ? ns[d] : 1);
+  }
+  std::vector<double> tab(pos, 1.0);
+  for (int d = 0; d < g.ndim; ++d) {
+    const long double L = g.Ls[d];
+    for (int k = 0; k < nterms; ++k) {
+      const ptf_flow_term& tm = terms[k];
+      const int fn = d == 0 ? tm.fx : (d == 1 ? tm.fy : tm.fz);
+      const double kk = d == 0 ? tm.kx : (d == 1 ? tm.ky : tm.kz);
+      const double ph = d == 0 ? tm.px : (d == 1 ? tm.py : tm.pz);
+      for (int j = 0; j < ns[d]; ++j) {
+        const double x = (double)(-L / 2 + (long double)j * (L / ns[d]));
+        const double a = kk * x + ph;
+        tab[off[d] + (size_t)k * ns[d] + j] =
+            fn == PTF_FN_SIN ? std::sin(a) : (fn == PTF_FN_COS ? std::cos(a) : 1.0);
+      }
+    }
+  }
+  CHK(dalloc(g.atab, tab.size() * sizeof(double)));
+  HIPCHK(hipMemcpy(g.atab.p, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice));
+  g.aterms.assign(terms, terms + nterms);
+  AnalyticArgs<T> a;
+  std::memset(&a, 0, sizeof(a));
+  a.nterms = nterms;
+  for (int k = 0; k < nterms; ++k) {
+    a.comp[k] = terms[k].component;
+    a.amp[k] = terms[k].amp;
+  }
+  a.tx = (const double*)g.atab.p + off[0];
+  a.ty = (const double*)g.atab.p + off[1];
+  a.tz = (const double*)g.atab.p + off[2];
+  for (int c = 0; c < g.ndim; ++c) a.out[c] = (T*)g.flow[c].p;
+  a.ndim = g.ndim;
+  a.nx = g.nx;
+  a.ny = g.ny;
+  a.nz = g.nz;
+  a.lb = g.lb;
+  a.npoints = g.nR;
+  Slot s;
+  s.name = "flow_analytic_synthesis";
+  s.dbl = g.dbl;
+  s.kind = K_ANALYTIC;
+  s.fn = reinterpret_cast<const void*>(&analytic_flow_kernel<T>);
+  s.fixed_threads = 256;
+  s.fixed_blocks = (unsigned)std::min<long long>((g.nR + 255) / 256, 16384);
+  set_args(s, a);
+  bool tdep = false;
+  for (int k = 0; k < nterms; ++k) tdep = tdep || terms[k].time_fn != PTF_FN_ONE;
+  g.flowsyn.clear();
+  if (tdep) {
+    g.flowsyn.push_back(s);  // synthesised at clock.t before every step
+  } else {
+    CHK(launch(s, prob->stream));
+    HIPCHK(hipStreamSynchronize(prob->stream));
+  }
+  return PTF_OK;
+}
+
+extern "C" {
+
+int ptf_set_analytic_flow(ptf_problem* prob, const ptf_flow_term* terms, int32_t nterms) {
+  if (!prob || !terms) return fail(PTF_ERR_INVALID, "NULL argument");
+  if (prob->d.flow_kind != PTF_FLOW_ANALYTIC)
+    return fail(PTF_ERR_INVALID, "problem was not created with PTF_FLOW_ANALYTIC");
+  if (nterms < 1 || nterms > PTF_MAX_FLOW_TERMS)
+    return fail(PTF_ERR_INVALID, "nterms must be in [1, PTF_MAX_FLOW_TERMS]");
+  for (int k = 0; k < nterms; ++k) {
+    const ptf_flow_term& t = terms[k];
+    if (t.component < 0 || t.component >= prob->g.ndim)
+      return fail(PTF_ERR_INVALID, "term " + std::to_string(k) + ": component out of range");
+    for (int f : {t.fx, t.fy, t.fz, t.time_fn})
+      if (f < PTF_FN_ONE || f > PTF_FN_COS)
+        return fail(PTF_ERR_INVALID, "term " + std::to_string(k) + ": unknown function");
+    for (double v : {t.kx, t.ky, t.kz, t.px, t.py, t.pz, t.amp, t.omega, t.tphase})
+      if (!std::isfinite(v))
+        return fail(PTF_ERR_INVALID, "term " + std::to_string(k) + ": not finite");
+  }
+  prob->g.primed = false;
+  CHK(prob->g.dbl ? set_analytic_impl<double>(prob, terms, nterms)
+                  : set_analytic_impl<float>(prob, terms, nterms));
+  prob->flow_set = true;
+  return PTF_OK;
+}
+
 int ptf_set_c(ptf_problem* prob, const double* c) {
   if (!prob || !c) return fail(PTF_ERR_INVALID, "NULL argument");
   Group& g = prob->g;
@@ -1436,11 +1564,7 @@ int ptf_step_profiled(ptf_problem* prob, int64_t nsteps, double* slot_ms) {
   size_t k = 0;
   if (rc == PTF_OK && hipEventRecord(ev[k++], st) != hipSuccess) rc = fail(PTF_ERR_DEVICE, "hipEventRecord");
   for (int64_t i = 0; i < nsteps && rc == PTF_OK; ++i) {
-    if (!g.flowsyn.empty()) {
-      Slot& s0 = g.flowsyn[0];
-      if (g.dbl) args_of<ColArgs<double>>(s0).t = prob->t;
-      else args_of<ColArgs<float>>(s0).t = prob->t;
-    }
+    patch_flowsyn(g, prob->t);
     for (Slot* s : v) {
       if ((rc = launch(*s, st)) != PTF_OK) break;
       if (hipEventRecord(ev[k++], st) != hipSuccess) {
@@ -1517,8 +1641,8 @@ template <typename T>
 static int calcN_impl(ptf_problem* prob, const double* sol, double* N) {
   Group& g = prob->g;
   hipStream_t st = prob->stream;
-  if (g.flow_kind == PTF_FLOW_SPECTRAL) {
-    args_of<ColArgs<T>>(g.flowsyn[0]).t = prob->t;
+  if (!g.flowsyn.empty()) {
+    patch_flowsyn(g, prob->t);
     for (auto& fs : g.flowsyn) CHK(launch(fs, st));
   }
   g.primed = false;  // the evaluation below uses the transform scratch
@@ -1757,7 +1881,7 @@ int ptf_ensemble_create(const ptf_desc* descs, int32_t count, ptf_ensemble** out
   *out = nullptr;
   for (int i = 0; i < count; ++i) {
     CHK(validate_desc(&descs[i]));
-    if (descs[i].flow_kind == PTF_FLOW_SPECTRAL)
+    if (descs[i].flow_kind == PTF_FLOW_SPECTRAL || descs[i].flow_kind == PTF_FLOW_ANALYTIC)
       return fail(PTF_ERR_UNSUPPORTED, "ensemble members take steady or per-step flows");
     if (descs[i].stepper == PTF_FILTERED_RK4 || descs[i].stepper == PTF_FILTERED_ETDRK4)
       return fail(PTF_ERR_UNSUPPORTED, "ensemble members take the RK4 or ETDRK4 stepper");

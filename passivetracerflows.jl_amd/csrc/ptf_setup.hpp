@@ -122,6 +122,61 @@ __global__ void variance_partial_kernel(const cpx<T>* __restrict__ sol, long lon
   }
 }
 
+// Analytic separable flow at one time: out[c] at physical point (ix, iy, iz) is
+// sum over the terms k of component c of ((amp[k] tx[k][ix]) ty[k][iy]) tz[k][iz]
+// (amp already holds amp * T(omega t + phase)).  Values are stored in the
+// library's physical layout: x-pairs, with the last axis in bricks of 2^lb rows.
+constexpr int kMaxFlowTerms = 16;
+template <typename T>
+struct AnalyticArgs {
+  int nterms;
+  int comp[kMaxFlowTerms];
+  double amp[kMaxFlowTerms];
+  const double* tx;   // [nterms][nx]
+  const double* ty;   // [nterms][ny]
+  const double* tz;   // [nterms][nz]
+  T* out[3];
+  int ndim, nx, ny, nz, lb;
+  long long npoints;
+};
+
+template <typename T>
+__global__ void analytic_flow_kernel(AnalyticArgs<T> a) {
+  const long long B = 1ll << a.lb, mask = B - 1;
+  const long long mid = a.ndim == 3 ? a.ny : 1;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < a.npoints;
+       i += (long long)gridDim.x * blockDim.x) {
+    const int ix = (int)(i % a.nx);
+    const long long r = i / a.nx;          // row: iy + ny iz
+    const int iy = a.ndim >= 2 ? (int)(r % a.ny) : 0;
+    const int iz = a.ndim == 3 ? (int)(r / a.ny) : 0;
+    double v0 = 0.0, v1 = 0.0, v2 = 0.0;
+    for (int k = 0; k < a.nterms; ++k) {
+      double x = a.amp[k] * a.tx[(long long)k * a.nx + ix];
+      if (a.ndim >= 2) x *= a.ty[(long long)k * a.ny + iy];
+      if (a.ndim == 3) x *= a.tz[(long long)k * a.nz + iz];
+      const int c = a.comp[k];
+      if (c == 0) v0 += x;
+      else if (c == 1) v1 += x;
+      else v2 += x;
+    }
+    // element ix of row r: pair ix/2 (bricked rows), component ix & 1
+    long long e;
+    const long long rowlen = a.nx / 2;
+    if (a.lb == 0) {
+      e = r * rowlen + ix / 2;
+    } else {
+      const long long l = r / mid, m = r % mid;
+      const long long b = ((l >> a.lb) * mid + m) * B + (l & mask);
+      e = ((b >> a.lb) * rowlen + ix / 2) * B + (b & mask);
+    }
+    const long long o = 2 * e + (ix & 1);
+    a.out[0][o] = (T)v0;
+    if (a.ndim >= 2) a.out[1][o] = (T)v1;
+    if (a.ndim == 3) a.out[2][o] = (T)v2;
+  }
+}
+
 // dealias!: zero the spectral elements whose kr index lies in [rlo, rhi) or whose
 // y (z) index lies in [ylo, yhi) ([zlo, zhi)); plain (unbricked) half-spectrum
 // layout (nx/2+1, ny, nz), x fastest.

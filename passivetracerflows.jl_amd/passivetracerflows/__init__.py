@@ -8,6 +8,7 @@ libptf_hip (gfx950); this package is the host-side binding.
 from . import _lib
 from . import fourierflows as FourierFlows
 from . import traceradvectiondiffusion as TracerAdvectionDiffusion
+from . import flows
 from .ensemble import Ensemble
 from .output import Diagnostic, Output, increment_, jldopen, record_, save_records, saveoutput, saveproblem
 from .fourierflows import (GPU, Clock, DeviceArray, OneDGrid, ROCm, ThreeDGrid, TwoDGrid, dealias_,
@@ -18,7 +19,7 @@ from .traceradvectiondiffusion import (OneDAdvectingFlow, SpectralStreamfunction
 stepforward = stepforward_
 
 __all__ = [
-    "FourierFlows", "TracerAdvectionDiffusion", "Ensemble", "ROCm", "GPU", "Clock", "DeviceArray",
+    "flows", "FourierFlows", "TracerAdvectionDiffusion", "Ensemble", "ROCm", "GPU", "Clock", "DeviceArray",
     "OneDGrid", "TwoDGrid", "ThreeDGrid", "gridpoints", "mul_", "ldiv_", "dealias_", "stepforward_", "step_until_",
     "OneDAdvectingFlow", "TwoDAdvectingFlow", "ThreeDAdvectingFlow", "SpectralStreamfunctionFlow",
     "set_c_", "updatevars_", "Diagnostic", "increment_", "Output", "saveproblem", "saveoutput",

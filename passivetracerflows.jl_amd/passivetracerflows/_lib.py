@@ -17,7 +17,7 @@ PTF_FP64, PTF_MIXED = 0, 1
 PTF_RK4, PTF_ETDRK4, PTF_FILTERED_RK4, PTF_FILTERED_ETDRK4 = 0, 1, 2, 3
 STEPPERS = {"RK4": PTF_RK4, "ETDRK4": PTF_ETDRK4, "FILTEREDRK4": PTF_FILTERED_RK4,
             "FILTEREDETDRK4": PTF_FILTERED_ETDRK4}
-PTF_FLOW_STEADY, PTF_FLOW_PER_STEP, PTF_FLOW_SPECTRAL = 0, 1, 2
+PTF_FLOW_STEADY, PTF_FLOW_PER_STEP, PTF_FLOW_SPECTRAL, PTF_FLOW_ANALYTIC = 0, 1, 2, 3
 
 
 class PTFError(RuntimeError):
@@ -30,6 +30,16 @@ class PTFError(RuntimeError):
 
 class LibraryNotBuilt(ImportError):
     pass
+
+
+class ptf_flow_term(C.Structure):
+    _fields_ = [
+        ("component", C.c_int32), ("fx", C.c_int32), ("fy", C.c_int32), ("fz", C.c_int32),
+        ("kx", C.c_double), ("ky", C.c_double), ("kz", C.c_double),
+        ("px", C.c_double), ("py", C.c_double), ("pz", C.c_double),
+        ("amp", C.c_double), ("time_fn", C.c_int32), ("reserved", C.c_int32),
+        ("omega", C.c_double), ("tphase", C.c_double),
+    ]
 
 
 class ptf_desc(C.Structure):
@@ -78,6 +88,7 @@ SIGNATURES = {
     "ptf_problem_get_stream": [_VP, C.POINTER(_VP)],
     "ptf_set_flow": [_VP, _DP, _DP, _DP],
     "ptf_set_spectral_flow": [_VP, _DP, _DP],
+    "ptf_set_analytic_flow": [_VP, C.POINTER(ptf_flow_term), _I32],
     "ptf_set_c": [_VP, _DP],
     "ptf_step": [_VP, _I64],
     "ptf_get_clock": [_VP, _DP, C.POINTER(_I64)],

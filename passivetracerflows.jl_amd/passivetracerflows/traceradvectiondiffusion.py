@@ -17,8 +17,8 @@ FP64 setup coefficients and diagnostics).  The stepping, transforms and
 calcN! all run in libptf_hip on the GPU; there is no CPU path.
 
 Additions without a reference counterpart: ``SpectralStreamfunctionFlow``
-(a built-in time-varying flow synthesised on the GPU every step),
-``mass``/``variance`` (FP64 diagnostics).
+and the analytic flows of ``passivetracerflows.flows`` (built-in flows
+synthesised on the GPU), ``mass``/``variance`` (FP64 diagnostics).
 """
 from __future__ import annotations
 
@@ -137,6 +137,16 @@ class ConstDiffTimeVaryingFlowParams3D(_Params):
 class ConstDiffSpectralFlowParams2D(_Params):
     ndim = 2
     steady = False
+
+
+class ConstDiffAnalyticFlowParams(_Params):
+    """Diffusivities plus an AnalyticAdvectingFlow (synthesised on the GPU)."""
+    steady = False
+
+    def __init__(self, kappa, eta, iota, kappah, nkappah, flow):
+        super().__init__(kappa, eta, iota, kappah, nkappah, ())
+        self.flow = flow
+        self.ndim = flow.ndim
 
 
 def _sample(f, pts, shape):
@@ -271,7 +281,24 @@ def Problem(dev, advecting_flow, **kw):
     stepper = kw.pop("stepper", "RK4")
     T = kw.pop("T", np.float64)
     kappa = _kw(kw, "κ", "kappa", 0.1)
-    if isinstance(advecting_flow, OneDAdvectingFlow):
+    from .flows import AnalyticAdvectingFlow
+    if isinstance(advecting_flow, AnalyticAdvectingFlow):
+        nd = advecting_flow.ndim
+        ny = kw.pop("ny", nx) if nd >= 2 else 1
+        Ly = kw.pop("Ly", Lx) if nd >= 2 else 1.0
+        nz = kw.pop("nz", nx) if nd == 3 else 1
+        Lz = kw.pop("Lz", Lx) if nd == 3 else 1.0
+        eta = _kw(kw, "η", "eta", kappa) if nd >= 2 else kappa
+        iota = _kw(kw, "ι", "iota", kappa) if nd == 3 else kappa
+        kappah = _kw(kw, "κh", "kappah", 0.0)
+        nkappah = _kw(kw, "nκh", "nkappah", 0)
+        if kw:
+            raise TypeError(f"unexpected keywords: {sorted(kw)}")
+        grid = (OneDGrid(dev, nx=nx, Lx=Lx, T=T) if nd == 1 else
+                TwoDGrid(dev, nx=nx, Lx=Lx, ny=ny, Ly=Ly, T=T) if nd == 2 else
+                ThreeDGrid(dev, nx=nx, Lx=Lx, ny=ny, Ly=Ly, nz=nz, Lz=Lz, T=T))
+        params = ConstDiffAnalyticFlowParams(kappa, eta, iota, kappah, nkappah, advecting_flow)
+    elif isinstance(advecting_flow, OneDAdvectingFlow):
         if kw:
             raise TypeError(f"unexpected keywords for a 1D problem: {sorted(kw)}")
         grid = OneDGrid(dev, nx=nx, Lx=Lx, T=T)
@@ -321,12 +348,15 @@ def Problem(dev, advecting_flow, **kw):
 # device problem creation and flow upload (called by fourierflows.Problem)
 # ---------------------------------------------------------------------------
 def _is_host_time_varying(params):
-    return (not params.steady) and not isinstance(params, ConstDiffSpectralFlowParams2D)
+    return (not params.steady) and not isinstance(params, (ConstDiffSpectralFlowParams2D,
+                                                           ConstDiffAnalyticFlowParams))
 
 
 def _flow_kind(params):
     if isinstance(params, ConstDiffSpectralFlowParams2D):
         return _lib.PTF_FLOW_SPECTRAL
+    if isinstance(params, ConstDiffAnalyticFlowParams):
+        return _lib.PTF_FLOW_ANALYTIC
     return _lib.PTF_FLOW_STEADY if params.steady else _lib.PTF_FLOW_PER_STEP
 
 
@@ -372,6 +402,12 @@ def _create_handle(prob, stepper, dt, filterkw=None):
             if f.psih0.shape != grid.spec_shape or f.omega.shape != grid.spec_shape:
                 raise ValueError("psih0/omega must have the spectral shape of the grid")
             call("ptf_set_spectral_flow", h, dptr(f.psih0.view(np.float64)), dptr(f.omega))
+        elif isinstance(params, ConstDiffAnalyticFlowParams):
+            if params.flow.ndim != grid.ndim:
+                raise ValueError("the analytic flow's dimension differs from the grid's")
+            terms = (_lib.ptf_flow_term * len(params.flow.terms))(
+                *[t.ctype() for t in params.flow.terms])
+            call("ptf_set_analytic_flow", h, terms, len(params.flow.terms))
         elif params.steady:
             _upload_fields(h, [getattr(params, n) for n in ("u", "v", "w")[: grid.ndim]])
         prob.vars._prob = prob
