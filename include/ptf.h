@@ -116,6 +116,15 @@ int ptf_buffer_download(void* host, const void* dptr, uint64_t nbytes);
 /* ---- FFT plans: grid.rfftplan with mul!/ldiv! (S:20, S:698-783, S:818, S:847) */
 /* ndim/n as the grid; precision PTF_FP64 (double) or PTF_MIXED (float).        */
 int ptf_plan_create(int32_t ndim, const int64_t n[3], int32_t precision, ptf_plan** out);
+/* A plan over `batch` consecutive fields of the same grid, transformed
+ * independently in one launch per pass: the layered (nx, ny, nlayers)
+ * transforms over dims 1:2 of the MQG-coupled tracer (Vars at S:678-679,
+ * fwdtransform! at S:865-866) are ndim = 2, batch = nlayers.  Member b of the
+ * real input/output starts at b*nx*ny*nz reals, of the spectral one at
+ * b*(nx/2+1)*ny*nz complex values; ptf_rfft/ptf_irfft/ptf_dealias act on all
+ * members. */
+int ptf_plan_create_batched(int32_t ndim, const int64_t n[3], int64_t batch, int32_t precision,
+                            ptf_plan** out);
 int ptf_plan_destroy(ptf_plan* plan);
 int ptf_plan_set_stream(ptf_plan* plan, void* hip_stream);
 /* mul!(out, rfftplan, in): unnormalised R2C; device pointers; input preserved. */

@@ -295,6 +295,7 @@ struct Group {
   std::vector<Slot> stages[4];
   std::vector<Slot> flowsyn;
   std::vector<ptf_flow_term> aterms;   // PTF_FLOW_ANALYTIC
+  bool plan_only = false;              // a transform plan: no state, scratch for B members
   DevBuf atab;                         // x, y, z factor tables of the terms
   // w0/w1 hold the first inverse pass of the current state (the fused last
   // kernel of each step leaves them so); anything else that writes sol or the
@@ -661,31 +662,33 @@ static int build_group_slots(Group& g) {
       g.flowsyn.push_back(s2);
     }
   }
-  // plan slots (single member, pointers patched per call)
-  g.p_r2c = make_slot(g, K_ROW_R2C, 0, 1);
+  // plan slots: all members of a batched plan; one member of a problem or ensemble
+  // group (pointers patched per call)
+  const int pm = g.plan_only ? g.B : 1;
+  g.p_r2c = make_slot(g, K_ROW_R2C, 0, pm);
   set_args(g.p_r2c, row_base<T>(g));
   g.p_r2c.name = "plan_x_r2c";
   CHK(pick_default(g.p_r2c));
-  g.p_c2r = make_slot(g, K_ROW_C2R, 0, 1);
+  g.p_c2r = make_slot(g, K_ROW_C2R, 0, pm);
   set_args(g.p_c2r, row_base<T>(g));
   g.p_c2r.name = "plan_x_c2r";
   CHK(pick_default(g.p_c2r));
   if (g.ndim >= 2) {
-    g.p_yf = make_slot(g, K_COL_FWD, 1, 1);
+    g.p_yf = make_slot(g, K_COL_FWD, 1, pm);
     set_args(g.p_yf, col_base<T>(g, 1));
     g.p_yf.name = "plan_y_fwd";
     CHK(pick_default(g.p_yf));
-    g.p_yi = make_slot(g, K_COL_INV, 1, 1);
+    g.p_yi = make_slot(g, K_COL_INV, 1, pm);
     set_args(g.p_yi, col_base<T>(g, 1));
     g.p_yi.name = "plan_y_inv";
     CHK(pick_default(g.p_yi));
   }
   if (g.ndim == 3) {
-    g.p_zf = make_slot(g, K_COL_FWD, 2, 1);
+    g.p_zf = make_slot(g, K_COL_FWD, 2, pm);
     set_args(g.p_zf, col_base<T>(g, 2));
     g.p_zf.name = "plan_z_fwd";
     CHK(pick_default(g.p_zf));
-    g.p_zi = make_slot(g, K_COL_INV, 2, 1);
+    g.p_zi = make_slot(g, K_COL_INV, 2, pm);
     set_args(g.p_zi, col_base<T>(g, 2));
     g.p_zi.name = "plan_z_inv";
     CHK(pick_default(g.p_zi));
@@ -755,24 +758,27 @@ static const double* spec_in(const Group& g, const double* src, std::vector<doub
   return tmp.data();
 }
 
-static int group_init(Group& g, const ptf_desc* descs, int count, bool allow_brick = true);
+static int group_init(Group& g, const ptf_desc* descs, int count, bool allow_brick = true,
+                      bool plan_only = false);
 
-static int group_init_once(Group& g, const ptf_desc* descs, int count, int lb);
+static int group_init_once(Group& g, const ptf_desc* descs, int count, int lb, bool plan_only);
 
-static int group_init(Group& g, const ptf_desc* descs, int count, bool allow_brick) {
+static int group_init(Group& g, const ptf_desc* descs, int count, bool allow_brick,
+                      bool plan_only) {
   const ptf_desc& d = descs[0];
   int lb = 0;
   if (allow_brick && d.ndim >= 2) {
     const long long nlast = d.ndim == 3 ? d.n[2] : d.n[1];
     if (nlast >= 64) lb = 3;
   }
-  int rc = group_init_once(g, descs, count, lb);
-  if (rc == PTF_ERR_UNSUPPORTED && lb > 0) rc = group_init_once(g, descs, count, 0);
+  int rc = group_init_once(g, descs, count, lb, plan_only);
+  if (rc == PTF_ERR_UNSUPPORTED && lb > 0) rc = group_init_once(g, descs, count, 0, plan_only);
   return rc;
 }
 
-static int group_init_once(Group& g, const ptf_desc* descs, int count, int lb) {
+static int group_init_once(Group& g, const ptf_desc* descs, int count, int lb, bool plan_only) {
   g.lb = lb;
+  g.plan_only = plan_only;
   const ptf_desc& d = descs[0];
   g.ndim = d.ndim;
   g.nx = (int)d.n[0];
@@ -817,6 +823,18 @@ static int group_init_once(Group& g, const ptf_desc* descs, int count, int lb) {
   const size_t Rb = (size_t)g.nR * g.B * g.esz;
   CHK(dalloc(g.d_ops, sizeof(OpParams) * count));
   HIPCHK(hipMemcpy(g.d_ops.p, g.ops.data(), sizeof(OpParams) * count, hipMemcpyHostToDevice));
+  if (plan_only) {
+    // a transform plan keeps only the inverse transform's scratch (all members)
+    CHK(dalloc(g.spec, S));
+    if (g.dbl) {
+      CHK(upload_twiddles<double>(g));
+      CHK(build_group_slots<double>(g));
+    } else {
+      CHK(upload_twiddles<float>(g));
+      CHK(build_group_slots<float>(g));
+    }
+    return PTF_OK;
+  }
   CHK(dalloc(g.sol, S));
   if (!g.dbl) {
     CHK(dalloc(g.sol_lo, S));
@@ -1278,8 +1296,14 @@ int ptf_buffer_download(void* host, const void* dptr, uint64_t nbytes) {
 
 // ----- plans -----------------------------------------------------------------
 int ptf_plan_create(int32_t ndim, const int64_t n[3], int32_t precision, ptf_plan** out) {
+  return ptf_plan_create_batched(ndim, n, 1, precision, out);
+}
+
+int ptf_plan_create_batched(int32_t ndim, const int64_t n[3], int64_t batch, int32_t precision,
+                            ptf_plan** out) {
   if (!out || !n) return fail(PTF_ERR_INVALID, "NULL argument");
   *out = nullptr;
+  if (batch < 1 || batch > 65535) return fail(PTF_ERR_INVALID, "batch must be in [1, 65535]");
   ptf_desc d;
   std::memset(&d, 0, sizeof(d));
   d.ndim = ndim;
@@ -1291,7 +1315,8 @@ int ptf_plan_create(int32_t ndim, const int64_t n[3], int32_t precision, ptf_pla
   d.dt = 1.0;
   CHK(validate_desc(&d));
   std::unique_ptr<ptf_plan> p(new ptf_plan());
-  CHK(group_init(p->g, &d, 1, /*allow_brick=*/false));
+  std::vector<ptf_desc> ds((size_t)batch, d);
+  CHK(group_init(p->g, ds.data(), (int)batch, /*allow_brick=*/false, /*plan_only=*/true));
   HIPCHK(hipStreamCreate(&p->stream));
   p->own_stream = true;
   *out = p.release();
@@ -1342,15 +1367,15 @@ int ptf_dealias(ptf_plan* plan, void* spec, double aliased_fraction) {
   rhi = g.nxh;
   if (g.ndim >= 2) aliased_range(g.ny, aliased_fraction, &ylo, &yhi);
   if (g.ndim == 3) aliased_range(g.nz, aliased_fraction, &zlo, &zhi);
-  const long long n = g.nS;
+  const long long n = g.nS * g.B;
   const int th = 256;
   const unsigned blocks = (unsigned)std::min<long long>((n + th - 1) / th, 65536);
   if (g.dbl)
     hipLaunchKernelGGL(dealias_kernel<double>, dim3(blocks), dim3(th), 0, plan->stream,
-                       (cpx<double>*)spec, n, g.nxh, g.ny, rlo, rhi, ylo, yhi, zlo, zhi);
+                       (cpx<double>*)spec, n, g.nxh, g.ny, g.nz, rlo, rhi, ylo, yhi, zlo, zhi);
   else
     hipLaunchKernelGGL(dealias_kernel<float>, dim3(blocks), dim3(th), 0, plan->stream,
-                       (cpx<float>*)spec, n, g.nxh, g.ny, rlo, rhi, ylo, yhi, zlo, zhi);
+                       (cpx<float>*)spec, n, g.nxh, g.ny, g.nz, rlo, rhi, ylo, yhi, zlo, zhi);
   HIPCHK(hipGetLastError());
   return PTF_OK;
 }

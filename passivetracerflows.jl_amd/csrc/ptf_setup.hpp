@@ -179,16 +179,16 @@ __global__ void analytic_flow_kernel(AnalyticArgs<T> a) {
 
 // dealias!: zero the spectral elements whose kr index lies in [rlo, rhi) or whose
 // y (z) index lies in [ylo, yhi) ([zlo, zhi)); plain (unbricked) half-spectrum
-// layout (nx/2+1, ny, nz), x fastest.
+// layout (nx/2+1, ny, nz, batch), x fastest.
 template <typename T>
-__global__ void dealias_kernel(cpx<T>* __restrict__ f, long long n, int nxh, int ny, int rlo,
-                               int rhi, int ylo, int yhi, int zlo, int zhi) {
+__global__ void dealias_kernel(cpx<T>* __restrict__ f, long long n, int nxh, int ny, int nz,
+                               int rlo, int rhi, int ylo, int yhi, int zlo, int zhi) {
   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (long long)gridDim.x * blockDim.x) {
     const int kx = (int)(i % nxh);
     const long long r = i / nxh;
     const int iy = (int)(r % ny);
-    const int iz = (int)(r / ny);
+    const int iz = (int)((r / ny) % nz);
     if ((kx >= rlo && kx < rhi) || (iy >= ylo && iy < yhi) || (iz >= zlo && iz < zhi))
       f[i] = cpx<T>{(T)0, (T)0};
   }

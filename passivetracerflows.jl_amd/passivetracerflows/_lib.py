@@ -78,6 +78,7 @@ SIGNATURES = {
     "ptf_buffer_upload": [_VP, _VP, C.c_uint64],
     "ptf_buffer_download": [_VP, _VP, C.c_uint64],
     "ptf_plan_create": [_I32, C.POINTER(_I64), _I32, C.POINTER(_VP)],
+    "ptf_plan_create_batched": [_I32, C.POINTER(_I64), _I64, _I32, C.POINTER(_VP)],
     "ptf_plan_destroy": [_VP],
     "ptf_plan_set_stream": [_VP, _VP],
     "ptf_rfft": [_VP, _VP, _VP],

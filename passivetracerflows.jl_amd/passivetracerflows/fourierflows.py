@@ -196,13 +196,22 @@ def _devptr(a):
 
 
 class RFFTPlan:
-    """grid.rfftplan: real<->complex FFT over all grid axes on the device."""
+    """grid.rfftplan: real<->complex FFT over all grid axes on the device.
 
-    def __init__(self, grid):
+    ``nlayers > 1`` makes the layered plan of the multi-layer tracer (the
+    ``(nx, ny, nlayers)`` transforms over dims 1:2 at S:678-679, S:865-866): arrays
+    hold nlayers fields back to back (numpy shape ``(nlayers,) + grid.shape``) and
+    every layer is transformed independently in one launch per pass."""
+
+    def __init__(self, grid, nlayers: int = 1):
         self.grid = grid
+        self.nlayers = int(nlayers)
         n = (C.c_int64 * 3)(grid.nx, grid.ny, grid.nz)
         h = C.c_void_p()
-        call("ptf_plan_create", grid.ndim, n, grid.precision, C.byref(h))
+        if self.nlayers == 1:
+            call("ptf_plan_create", grid.ndim, n, grid.precision, C.byref(h))
+        else:
+            call("ptf_plan_create_batched", grid.ndim, n, self.nlayers, grid.precision, C.byref(h))
         self._h = h.value
 
     def __del__(self):
@@ -226,10 +235,12 @@ def ldiv_(out, plan: RFFTPlan, a):
     return out
 
 
-def dealias_(fh, grid):
+def dealias_(fh, grid, plan: RFFTPlan | None = None):
     """dealias!(fh, grid): zero the aliased wavenumbers of the half spectrum fh (a device
-    array; grid.aliased_fraction, default 1/3 = the 2/3 rule)."""
-    call("ptf_dealias", grid.rfftplan._h, _devptr(fh), float(grid.aliased_fraction))
+    array; grid.aliased_fraction, default 1/3 = the 2/3 rule).  Pass the layered plan
+    for an (nlayers, ...) array."""
+    plan = grid.rfftplan if plan is None else plan
+    call("ptf_dealias", plan._h, _devptr(fh), float(grid.aliased_fraction))
     call("ptf_device_synchronize")
     return None
 

@@ -105,3 +105,38 @@ def test_linear_operator_vs_oracle(dev):
     ptf._lib.call("ptf_get_L", pr._h, L.ctypes.data_as(C.POINTER(C.c_double)))
     ref = O.linear_operator(g, 0.3, 0.2, 0.1)
     assert rel(L, ref) < 1e-15
+
+
+# layered (batched) plans: the (nx, ny, nlayers) transforms over dims 1:2 of the
+# multi-layer tracer (S:678-679, S:865-866), and batched 1D/3D fields
+LAYERED = [(2, (64, 32), 2), (2, (128, 128), 5), (1, (256,), 7), (3, (16, 32, 8), 3)]
+
+
+@pytest.mark.parametrize("ndim,n,nl", LAYERED)
+@pytest.mark.parametrize("T", [np.float64, np.float32])
+def test_layered_plan_vs_pocketfft(dev, ndim, n, nl, T):
+    print(f"layered ndim={ndim} n={n} nlayers={nl} T={T.__name__}", flush=True)
+    g = _grid(dev, ndim, n, T)
+    plan = ptf.FourierFlows.RFFTPlan(g, nlayers=nl)
+    rng = np.random.default_rng(2)
+    c = rng.standard_normal((nl,) + g.shape)
+    axes = tuple(range(1, ndim + 1))
+    ref = np.fft.rfftn(c, axes=axes)
+    ct = {np.float64: np.complex128, np.float32: np.complex64}[T]
+    din = ptf.DeviceArray.from_host(c.astype(T))
+    dout = ptf.DeviceArray((nl,) + g.spec_shape, ct)
+    ptf.mul_(dout, plan, din)
+    tol = 1e-13 if T is np.float64 else 2e-6
+    got = dout.to_host()
+    for layer in range(nl):          # each layer on its own, none mixed into another
+        assert rel(got[layer], ref[layer]) < tol
+    back = ptf.DeviceArray((nl,) + g.shape, T)
+    ptf.ldiv_(back, plan, dout)
+    assert rel(back.to_host(), c) < tol
+    # dealias! on every layer
+    ptf.dealias_(dout, g, plan)
+    og = O.Grid(ndim, n[0]) if ndim == 1 else (O.Grid(2, n[0], ny=n[1]) if ndim == 2
+                                              else O.Grid(3, n[0], ny=n[1], nz=n[2]))
+    d = dout.to_host()
+    for layer in range(nl):
+        assert np.array_equal(d[layer], O.dealias(og, got[layer]))
