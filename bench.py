@@ -208,6 +208,24 @@ def headline(dev, steps, warmup, T=np.float64, autotune=True):
     return cfg, prob, secs, (m0, m1, v0, v1)
 
 
+def repeat_and_pcie(prob, cfg, steps):
+    """Two more timed segments of `steps` steps (run-to-run spread), and one segment
+    that includes the host boundary: upload of u, v, w and c0 (ptf_set_flow, set_c!),
+    the steps, and the download of c (updatevars!)."""
+    import passivetracerflows as ptf
+    from passivetracerflows import traceradvectiondiffusion as tad
+    reps = [time_steps(lambda k: ptf.stepforward_(prob, k), steps) for _ in range(2)]
+    sync()
+    t0 = time.perf_counter()
+    tad._upload_fields(prob._h, list(cfg["flow"]))
+    tad.set_c_(prob, cfg["c0"])
+    ptf.stepforward_(prob, steps)
+    tad.updatevars_(prob)
+    sync()
+    pcie = time.perf_counter() - t0
+    return reps, pcie
+
+
 def slot_table(prob, nS, nR, cb, rb, steps):
     """Per-launch durations measured inside `steps` running steps (a HIP event around
     every launch on the problem's stream), with each launch's algorithmic bytes."""
@@ -379,6 +397,7 @@ def main():
     cfg, prob, secs, (m0, m1, v0, v1) = headline(dev, args.steps, args.warmup,
                                                  autotune=not args.no_autotune)
     s64 = prob.sol            # state after warmup + steps, for the mixed-form comparison
+    reps, pcie_secs = repeat_and_pcie(prob, cfg, args.steps)
     l1 = float(np.abs(cfg["c0"]).sum())
     npts = 256 ** 3
     value = gps(npts, args.steps, secs)
@@ -410,7 +429,13 @@ def main():
                                pass_model_GBs=PASS_MODEL_B_PER_GPS * value / 1e9,
                                pass_model_frac=PASS_MODEL_B_PER_GPS * value / 1e9 / HBM_PEAK_GBS,
                                mass_drift_rel_to_l1=abs(m1 - m0) / l1,
-                               variance_ratio=v1 / v0),
+                               variance_ratio=v1 / v0,
+                               repeat_ms_per_step=[1e3 * r / args.steps for r in reps],
+                               median_value=float(np.median([gps(npts, args.steps, r)
+                                                             for r in [secs] + reps]))),
+        "abc3d_256_fp64_pcie_inclusive": dict(
+            value=gps(npts, args.steps, pcie_secs), ms_per_step=1e3 * pcie_secs / args.steps,
+            steps=args.steps, includes="ptf_set_flow(u,v,w) + set_c! + steps + updatevars!"),
         "abc3d_256_mixed": dict(value=gps(npts, args.steps, secs32),
                                 ms_per_step=1e3 * secs32 / args.steps,
                                 relL2_vs_fp64_after_steps=mixed_err,

@@ -368,8 +368,10 @@ __device__ __forceinline__ Rk4Ops<T> rk4_load(const StageArgs<T>& a, long long u
   } else {
     o.x = (a.stg + ubase)[off];
     o.s = (a.sol + ubase)[off];
-    if (a.stage == 2 || a.sol_lo) o.ac = (a.acc + ubase)[off];
-    if (a.stage == 3 && a.sol_lo) o.lo = (a.sol_lo + ubase)[off];
+    // the compensated low part exists only in the mixed (FP32) form
+    constexpr bool kMixed = sizeof(T) == 4;
+    if (a.stage == 2 || (kMixed && a.sol_lo)) o.ac = (a.acc + ubase)[off];
+    if (kMixed && a.stage == 3 && a.sol_lo) o.lo = (a.sol_lo + ubase)[off];
   }
   return o;
 }
@@ -398,11 +400,11 @@ __device__ __forceinline__ cpx<T> rk4_finish(const StageArgs<T>& a, long long ub
     const cpx<T> nx = {o.s.re + dt * rhs.re, o.s.im + dt * rhs.im};
     (a.stg + ubase)[off] = nx;
     const cpx<T> tot = {o.ac.re + rhs.re / (T)3, o.ac.im + rhs.im / (T)3};
-    if (a.sol_lo) (a.acc + ubase)[off] = tot;
+    if (sizeof(T) == 4 && a.sol_lo) (a.acc + ubase)[off] = tot;
     else (a.sol + ubase)[off] = {o.s.re + dt * tot.re, o.s.im + dt * tot.im};
     return nx;
   } else {
-    if (!a.sol_lo) {
+    if (sizeof(T) == 8 || !a.sol_lo) {
       // o.s = sol + dt (RHS_1/6 + RHS_2/3 + RHS_3/3), stored by stage 3
       cpx<T> ns = {o.s.re + dt * (rhs.re / (T)6), o.s.im + dt * (rhs.im / (T)6)};
       if (a.filtered) {
