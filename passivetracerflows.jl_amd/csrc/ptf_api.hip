@@ -289,6 +289,7 @@ struct Group {
 
   std::vector<double> L_host;  // L in the public layout, all members
   DevBuf d_ops, sol, sol_lo, stg, stg2, acc, n1, n2, w[3], flow[3], d_L, etd[6], tw[3], psih0, omega;
+  DevBuf stage;   // FP64 staging of one member in the public layout (host transfers)
   DevBuf phys, spec, red, backup;  // set_c/updatevars/calcN staging, reductions, timing backup
   DevBuf fs[2];                    // spectral-flow synthesis scratch
   std::vector<Slot> prologue;      // first inverse pass of a step when w0/w1 are not primed
@@ -735,29 +736,6 @@ static void to_brick(const Group& g, const double* src, double* dst, size_t rowl
     }
 }
 
-static void from_brick(const Group& g, const double* src, double* dst, size_t rowlen, int w) {
-  const size_t rows = member_rows(g);
-  for (size_t r = 0; r < rows; ++r)
-    for (size_t k = 0; k < rowlen; ++k) {
-      const size_t d = brick_index(g, r, k, rowlen);
-      for (int e = 0; e < w; ++e) dst[(r * rowlen + k) * w + e] = src[d * w + e];
-    }
-}
-
-// host FP64 arrays of one member in the public layout -> the group's layout (and back)
-static const double* phys_in(const Group& g, const double* src, std::vector<double>& tmp) {
-  if (!g.lb) return src;
-  tmp.resize((size_t)g.nR);
-  to_brick(g, src, tmp.data(), (size_t)g.nx / 2, 2);
-  return tmp.data();
-}
-static const double* spec_in(const Group& g, const double* src, std::vector<double>& tmp, int w) {
-  if (!g.lb) return src;
-  tmp.resize((size_t)g.nS * w);
-  to_brick(g, src, tmp.data(), (size_t)g.nxh, w);
-  return tmp.data();
-}
-
 static int group_init(Group& g, const ptf_desc* descs, int count, bool allow_brick = true,
                       bool plan_only = false);
 
@@ -978,32 +956,6 @@ static int g_irfft(Group& g, const void* in, void* out, hipStream_t st) {
   return g.dbl ? group_irfft<double>(g, in, out, st) : group_irfft<float>(g, in, out, st);
 }
 
-// host FP64 <-> device working precision
-static int upload_real(Group& g, void* dst, const double* src, size_t n, hipStream_t st) {
-  if (g.dbl) {
-    HIPCHK(hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyHostToDevice, st));
-  } else {
-    std::vector<float> tmp(n);
-    for (size_t i = 0; i < n; ++i) tmp[i] = (float)src[i];
-    HIPCHK(hipMemcpyAsync(dst, tmp.data(), n * sizeof(float), hipMemcpyHostToDevice, st));
-    HIPCHK(hipStreamSynchronize(st));
-  }
-  return PTF_OK;
-}
-
-static int download_real(Group& g, double* dst, const void* src, size_t n, hipStream_t st) {
-  if (g.dbl) {
-    HIPCHK(hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyDeviceToHost, st));
-    HIPCHK(hipStreamSynchronize(st));
-  } else {
-    std::vector<float> tmp(n);
-    HIPCHK(hipMemcpyAsync(tmp.data(), src, n * sizeof(float), hipMemcpyDeviceToHost, st));
-    HIPCHK(hipStreamSynchronize(st));
-    for (size_t i = 0; i < n; ++i) dst[i] = (double)tmp[i];
-  }
-  return PTF_OK;
-}
-
 // time factor T(omega t + phase) of an analytic flow term
 static double term_time_factor(const ptf_flow_term& k, double t) {
   const double a = k.omega * t + k.tphase;
@@ -1029,6 +981,61 @@ static void patch_flowsyn(Group& g, double t) {
   } else {
     args_of<ColArgs<float>>(s).t = t;
   }
+}
+
+// ---- transfers in the public layout, converted on the GPU --------------------------
+// One member's field crosses PCIe in the reference layout (FP64); the brick layout
+// and the working precision are applied on the device.  rowlen = entries per x-row
+// (nx/2 x-pairs of a physical field, nx/2+1 spectral values), w = doubles per entry.
+static int ensure_stage(Group& g) {
+  const size_t need = (size_t)std::max<long long>(g.nR, 2 * g.nS) * sizeof(double);
+  if (!g.stage.p || g.stage.n < need) CHK(dalloc(g.stage, need));
+  return PTF_OK;
+}
+
+static unsigned layout_blocks(long long n) {
+  return (unsigned)std::min<long long>((n + 255) / 256, 16384);
+}
+
+// host public FP64 -> device working layout (dst, and dst_lo for a hi + lo split);
+// dbl selects the device element type (FP64 fields of the mixed form pass true)
+static int put_public(Group& g, bool dbl, void* dst, void* dst_lo, const double* src,
+                      long long rowlen, int w, hipStream_t st) {
+  CHK(ensure_stage(g));
+  const long long rows = (long long)g.ny * g.nz;
+  const long long n = rows * rowlen * w;
+  HIPCHK(hipMemcpyAsync(g.stage.p, src, (size_t)n * sizeof(double), hipMemcpyHostToDevice, st));
+  if (dbl)
+    hipLaunchKernelGGL(layout_in_kernel<double>, dim3(layout_blocks(n)), dim3(256), 0, st,
+                       (const double*)g.stage.p, (double*)dst, (double*)dst_lo, rows, (int)rowlen, w,
+                       g.lb, g.ndim, g.ny);
+  else
+    hipLaunchKernelGGL(layout_in_kernel<float>, dim3(layout_blocks(n)), dim3(256), 0, st,
+                       (const double*)g.stage.p, (float*)dst, (float*)dst_lo, rows, (int)rowlen, w,
+                       g.lb, g.ndim, g.ny);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(st));
+  return PTF_OK;
+}
+
+// device working layout (src, plus src_lo) -> host public FP64
+static int get_public(Group& g, bool dbl, double* dst, const void* src, const void* src_lo,
+                      long long rowlen, int w, hipStream_t st) {
+  CHK(ensure_stage(g));
+  const long long rows = (long long)g.ny * g.nz;
+  const long long n = rows * rowlen * w;
+  if (dbl)
+    hipLaunchKernelGGL(layout_out_kernel<double>, dim3(layout_blocks(n)), dim3(256), 0, st,
+                       (const double*)src, (const double*)src_lo, (double*)g.stage.p, rows,
+                       (int)rowlen, w, g.lb, g.ndim, g.ny);
+  else
+    hipLaunchKernelGGL(layout_out_kernel<float>, dim3(layout_blocks(n)), dim3(256), 0, st,
+                       (const float*)src, (const float*)src_lo, (double*)g.stage.p, rows,
+                       (int)rowlen, w, g.lb, g.ndim, g.ny);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(dst, g.stage.p, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  return PTF_OK;
 }
 
 static int group_step_launches(Group& g, hipStream_t st, double t) {
@@ -1184,53 +1191,17 @@ struct ptf_ensemble {
 
 // sol of one member as FP64 (hi + lo in the mixed form)
 static int get_member_sol(Group& g, int member, double* out_pub, hipStream_t st) {
-  const size_t n = (size_t)g.nS * 2;
-  std::vector<double> lay;
-  double* out = out_pub;
-  if (g.lb) {
-    lay.resize(n);
-    out = lay.data();
-  }
-  if (g.dbl) {
-    HIPCHK(hipMemcpyAsync(out, (const char*)g.sol.p + (size_t)member * n * 8, n * 8,
-                          hipMemcpyDeviceToHost, st));
-    HIPCHK(hipStreamSynchronize(st));
-  } else {
-    std::vector<float> hi(n), lo(n);
-    HIPCHK(hipMemcpyAsync(hi.data(), (const char*)g.sol.p + (size_t)member * n * 4, n * 4,
-                          hipMemcpyDeviceToHost, st));
-    HIPCHK(hipMemcpyAsync(lo.data(), (const char*)g.sol_lo.p + (size_t)member * n * 4, n * 4,
-                          hipMemcpyDeviceToHost, st));
-    HIPCHK(hipStreamSynchronize(st));
-    for (size_t i = 0; i < n; ++i) out[i] = (double)hi[i] + (double)lo[i];
-  }
-  if (g.lb) from_brick(g, out, out_pub, (size_t)g.nxh, 2);
-  return PTF_OK;
+  const size_t off = (size_t)member * g.nS * 2 * g.esz;
+  const void* lo = g.sol_lo.p ? (const char*)g.sol_lo.p + off : nullptr;
+  return get_public(g, g.dbl, out_pub, (const char*)g.sol.p + off, lo, g.nxh, 2, st);
 }
 
 // set sol of one member from FP64 (split into hi + lo in the mixed form)
 static int set_member_sol(Group& g, int member, const double* in_pub, hipStream_t st) {
-  const size_t n = (size_t)g.nS * 2;
   g.primed = false;
-  std::vector<double> tmp;
-  const double* in = spec_in(g, in_pub, tmp, 2);
-  if (g.dbl) {
-    HIPCHK(hipMemcpyAsync((char*)g.sol.p + (size_t)member * n * 8, in, n * 8,
-                          hipMemcpyHostToDevice, st));
-    HIPCHK(hipStreamSynchronize(st));
-    return PTF_OK;
-  }
-  std::vector<float> hi(n), lo(n);
-  for (size_t i = 0; i < n; ++i) {
-    hi[i] = (float)in[i];
-    lo[i] = (float)(in[i] - (double)hi[i]);
-  }
-  HIPCHK(hipMemcpyAsync((char*)g.sol.p + (size_t)member * n * 4, hi.data(), n * 4,
-                        hipMemcpyHostToDevice, st));
-  HIPCHK(hipMemcpyAsync((char*)g.sol_lo.p + (size_t)member * n * 4, lo.data(), n * 4,
-                        hipMemcpyHostToDevice, st));
-  HIPCHK(hipStreamSynchronize(st));
-  return PTF_OK;
+  const size_t off = (size_t)member * g.nS * 2 * g.esz;
+  void* lo = g.sol_lo.p ? (char*)g.sol_lo.p + off : nullptr;
+  return put_public(g, g.dbl, (char*)g.sol.p + off, lo, in_pub, g.nxh, 2, st);
 }
 
 // after a set_c the low part of the member's compensated state is zero
@@ -1425,9 +1396,7 @@ int ptf_set_flow(ptf_problem* prob, const double* u, const double* v, const doub
   Group& g = prob->g;
   for (int i = 0; i < g.ndim; ++i) {
     if (!f[i]) return fail(PTF_ERR_INVALID, "flow component " + std::to_string(i) + " is NULL");
-    std::vector<double> tmp;
-    CHK(upload_real(g, g.flow[i].p, phys_in(g, f[i], tmp), (size_t)g.nR, prob->stream));
-    HIPCHK(hipStreamSynchronize(prob->stream));
+    CHK(put_public(g, g.dbl, g.flow[i].p, nullptr, f[i], g.nx / 2, 2, prob->stream));
   }
   HIPCHK(hipStreamSynchronize(prob->stream));
   prob->flow_set = true;
@@ -1439,11 +1408,8 @@ int ptf_set_spectral_flow(ptf_problem* prob, const double* psih0, const double* 
   if (prob->d.flow_kind != PTF_FLOW_SPECTRAL)
     return fail(PTF_ERR_INVALID, "problem was not created with PTF_FLOW_SPECTRAL");
   Group& g = prob->g;
-  std::vector<double> t1, t2;
-  HIPCHK(hipMemcpy(g.psih0.p, spec_in(g, psih0, t1, 2), (size_t)g.nS * 2 * sizeof(double),
-                   hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(g.omega.p, spec_in(g, omega, t2, 1), (size_t)g.nS * sizeof(double),
-                   hipMemcpyHostToDevice));
+  CHK(put_public(g, true, g.psih0.p, nullptr, psih0, g.nxh, 2, prob->stream));
+  CHK(put_public(g, true, g.omega.p, nullptr, omega, g.nxh, 1, prob->stream));
   prob->flow_set = true;
   return PTF_OK;
 }
@@ -1546,9 +1512,7 @@ int ptf_set_analytic_flow(ptf_problem* prob, const ptf_flow_term* terms, int32_t
 int ptf_set_c(ptf_problem* prob, const double* c) {
   if (!prob || !c) return fail(PTF_ERR_INVALID, "NULL argument");
   Group& g = prob->g;
-  std::vector<double> tmp;
-  CHK(upload_real(g, g.phys.p, phys_in(g, c, tmp), (size_t)g.nR, prob->stream));
-  HIPCHK(hipStreamSynchronize(prob->stream));
+  CHK(put_public(g, g.dbl, g.phys.p, nullptr, c, g.nx / 2, 2, prob->stream));
   CHK(g_rfft(g, g.phys.p, g.sol.p, prob->stream));
   CHK(clear_member_lo(g, 0, prob->stream));
   g.primed = false;
@@ -1632,13 +1596,6 @@ int ptf_set_clock(ptf_problem* prob, double t, int64_t step) {
   return PTF_OK;
 }
 
-static int download_spec(Group& g, double* dst, const void* src, hipStream_t st) {
-  return download_real(g, dst, src, (size_t)g.nS * 2, st);
-}
-static int upload_spec(Group& g, void* dst, const double* src, hipStream_t st) {
-  return upload_real(g, dst, src, (size_t)g.nS * 2, st);
-}
-
 int ptf_get_sol(ptf_problem* prob, double* sol) {
   if (!prob || !sol) return fail(PTF_ERR_INVALID, "NULL argument");
   return get_member_sol(prob->g, 0, sol, prob->stream);
@@ -1653,11 +1610,7 @@ int ptf_updatevars(ptf_problem* prob, double* c) {
   if (!prob || !c) return fail(PTF_ERR_INVALID, "NULL argument");
   Group& g = prob->g;
   CHK(g_irfft(g, g.sol.p, g.phys.p, prob->stream));
-  if (!g.lb) return download_real(g, c, g.phys.p, (size_t)g.nR, prob->stream);
-  std::vector<double> tmp((size_t)g.nR);
-  CHK(download_real(g, tmp.data(), g.phys.p, (size_t)g.nR, prob->stream));
-  from_brick(g, tmp.data(), c, (size_t)g.nx / 2, 2);
-  return PTF_OK;
+  return get_public(g, g.dbl, c, g.phys.p, nullptr, g.nx / 2, 2, prob->stream);
 }
 
 }  // extern "C"
@@ -1671,20 +1624,14 @@ static int calcN_impl(ptf_problem* prob, const double* sol, double* N) {
     for (auto& fs : g.flowsyn) CHK(launch(fs, st));
   }
   g.primed = false;  // the evaluation below uses the transform scratch
-  std::vector<double> tmp;
-  CHK(upload_spec(g, g.stg.p, spec_in(g, sol, tmp, 2), st));
-  HIPCHK(hipStreamSynchronize(st));
+  CHK(put_public(g, g.dbl, g.stg.p, nullptr, sol, g.nxh, 2, st));
   StageArgs<T> none;
   std::memset(&none, 0, sizeof(none));
   none.kind = EPI_STORE;
   std::vector<Slot> list;
   CHK(build_calcN<T>(g, (const cpx<T>*)g.stg.p, none, (cpx<T>*)g.acc.p, list, "calcN_"));
   for (auto& s : list) CHK(launch(s, st));
-  if (!g.lb) return download_spec(g, N, g.acc.p, st);
-  std::vector<double> out((size_t)g.nS * 2);
-  CHK(download_spec(g, out.data(), g.acc.p, st));
-  from_brick(g, out.data(), N, (size_t)g.nxh, 2);
-  return PTF_OK;
+  return get_public(g, g.dbl, N, g.acc.p, nullptr, g.nxh, 2, st);
 }
 
 extern "C" {
@@ -1816,16 +1763,10 @@ int ptf_step_record(ptf_problem* prob, int64_t nsteps, int64_t every, int64_t ca
     }
   }
   if (rc == PTF_OK && c_out) {
-    std::vector<double> tmp(g.lb ? (size_t)g.nR : 0);
     for (int64_t k = 0; k < nrec && rc == PTF_OK; ++k) {
       double* dst = c_out + (size_t)k * g.nR;
       const void* src = (const char*)snaps.p + (size_t)k * rsz;
-      if (!g.lb) {
-        rc = download_real(g, dst, src, (size_t)g.nR, st);
-      } else {
-        rc = download_real(g, tmp.data(), src, (size_t)g.nR, st);
-        if (rc == PTF_OK) from_brick(g, tmp.data(), dst, (size_t)g.nx / 2, 2);
-      }
+      rc = get_public(g, g.dbl, dst, src, nullptr, g.nx / 2, 2, st);
     }
   }
   if (rc == PTF_OK && hipStreamSynchronize(st) != hipSuccess)
@@ -2000,9 +1941,7 @@ int ptf_ensemble_set_flow(ptf_ensemble* ens, int32_t member, const double* u, co
   for (int c = 0; c < g->ndim && c < 2; ++c) {
     if (!f[c]) return fail(PTF_ERR_INVALID, "NULL flow component");
     char* dst = (char*)g->flow[c].p + (size_t)i * g->nR * g->esz;
-    std::vector<double> tmp;
-    CHK(upload_real(*g, dst, phys_in(*g, f[c], tmp), (size_t)g->nR, ens->stream));
-    HIPCHK(hipStreamSynchronize(ens->stream));
+    CHK(put_public(*g, g->dbl, dst, nullptr, f[c], g->nx / 2, 2, ens->stream));
   }
   HIPCHK(hipStreamSynchronize(ens->stream));
   return PTF_OK;
@@ -2013,9 +1952,7 @@ int ptf_ensemble_set_c(ptf_ensemble* ens, int32_t member, const double* c) {
   int i;
   CHK(member_of(ens, member, &g, &i));
   if (!c) return fail(PTF_ERR_INVALID, "c is NULL");
-  std::vector<double> tmp;
-  CHK(upload_real(*g, g->phys.p, phys_in(*g, c, tmp), (size_t)g->nR, ens->stream));
-  HIPCHK(hipStreamSynchronize(ens->stream));
+  CHK(put_public(*g, g->dbl, g->phys.p, nullptr, c, g->nx / 2, 2, ens->stream));
   char* dst = (char*)g->sol.p + (size_t)i * g->nS * 2 * g->esz;
   CHK(g_rfft(*g, g->phys.p, dst, ens->stream));
   CHK(clear_member_lo(*g, i, ens->stream));
@@ -2031,11 +1968,7 @@ int ptf_ensemble_get_c(ptf_ensemble* ens, int32_t member, double* c) {
   if (!c) return fail(PTF_ERR_INVALID, "c is NULL");
   char* src = (char*)g->sol.p + (size_t)i * g->nS * 2 * g->esz;
   CHK(g_irfft(*g, src, g->phys.p, ens->stream));
-  if (!g->lb) return download_real(*g, c, g->phys.p, (size_t)g->nR, ens->stream);
-  std::vector<double> tmp((size_t)g->nR);
-  CHK(download_real(*g, tmp.data(), g->phys.p, (size_t)g->nR, ens->stream));
-  from_brick(*g, tmp.data(), c, (size_t)g->nx / 2, 2);
-  return PTF_OK;
+  return get_public(*g, g->dbl, c, g->phys.p, nullptr, g->nx / 2, 2, ens->stream);
 }
 
 int ptf_ensemble_get_sol(ptf_ensemble* ens, int32_t member, double* sol) {

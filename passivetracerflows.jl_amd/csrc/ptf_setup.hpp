@@ -122,6 +122,57 @@ __global__ void variance_partial_kernel(const cpx<T>* __restrict__ sol, long lon
   }
 }
 
+// Element (row r, entry k) of a field in the working layout: rows are the (y, z)
+// lines of x (r = y + ny z); with the last axis in bricks of 2^lb planes the rows
+// are regrouped so that 2^lb consecutive planes sit innermost.
+__device__ __forceinline__ long long work_index(long long r, long long k, long long rowlen, int lb,
+                                                int ndim, int ny) {
+  if (lb == 0) return r * rowlen + k;
+  const long long B = 1ll << lb, mask = B - 1;
+  const long long mid = ndim == 3 ? ny : 1;
+  const long long l = r / mid, m = r % mid;
+  const long long b = ((l >> lb) * mid + m) * B + (l & mask);
+  return ((b >> lb) * rowlen + k) * B + (b & mask);
+}
+
+// Public FP64 layout (rows x rowlen entries of w doubles, x fastest) -> working
+// layout in T; with dst_lo the FP64 value is split into hi + lo (compensated state).
+template <typename T>
+__global__ void layout_in_kernel(const double* __restrict__ src, T* __restrict__ dst,
+                                 T* __restrict__ dst_lo, long long rows, int rowlen, int w, int lb,
+                                 int ndim, int ny) {
+  const long long n = rows * rowlen * w;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const int e = (int)(i % w);
+    const long long q = i / w;
+    const long long k = q % rowlen, r = q / rowlen;
+    const long long d = work_index(r, k, rowlen, lb, ndim, ny) * w + e;
+    const double x = src[i];
+    const T hi = (T)x;
+    dst[d] = hi;
+    if (dst_lo) dst_lo[d] = (T)(x - (double)hi);
+  }
+}
+
+// Working layout (T, plus the low part when src_lo) -> public FP64 layout.
+template <typename T>
+__global__ void layout_out_kernel(const T* __restrict__ src, const T* __restrict__ src_lo,
+                                  double* __restrict__ dst, long long rows, int rowlen, int w,
+                                  int lb, int ndim, int ny) {
+  const long long n = rows * rowlen * w;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const int e = (int)(i % w);
+    const long long q = i / w;
+    const long long k = q % rowlen, r = q / rowlen;
+    const long long d = work_index(r, k, rowlen, lb, ndim, ny) * w + e;
+    double x = (double)src[d];
+    if (src_lo) x += (double)src_lo[d];
+    dst[i] = x;
+  }
+}
+
 // Analytic separable flow at one time: out[c] at physical point (ix, iy, iz) is
 // sum over the terms k of component c of ((amp[k] tx[k][ix]) ty[k][iy]) tz[k][iz]
 // (amp already holds amp * T(omega t + phase)).  Values are stored in the
