@@ -161,6 +161,10 @@ int ptf_set_analytic_flow(ptf_problem* prob, const ptf_flow_term* terms, int32_t
 int ptf_set_c(ptf_problem* prob, const double* c);
 /* stepforward!(prob, nsteps) [FF]: clock.t += dt, clock.step += 1 after each step. */
 int ptf_step(ptf_problem* prob, int64_t nsteps);
+/* With a steady flow, ptf_step replays whole blocks of 8 steps from a captured
+ * hipGraph (kernels only; the captured arguments do not depend on the clock);
+ * enable = 0 launches every kernel directly.  Default: enabled. */
+int ptf_problem_set_graph(ptf_problem* prob, int32_t enable);
 int ptf_get_clock(ptf_problem* prob, double* t, int64_t* step);
 int ptf_set_clock(ptf_problem* prob, double t, int64_t step);
 /* prob.sol read/write (host interleaved FP64 S-field). */

@@ -1157,10 +1157,22 @@ struct ptf_problem {
   hipStream_t stream = nullptr;
   bool own_stream = false;
   bool flow_set = false;
+  // replay of kGraphSteps captured steps (steady flows; rebuilt after variant changes)
+  bool use_graph = true;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  void drop_graph() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    exec = nullptr;
+    graph = nullptr;
+  }
   ~ptf_problem() {
+    drop_graph();
     if (own_stream && stream) (void)hipStreamDestroy(stream);
   }
 };
+static constexpr int kGraphSteps = 8;
 
 struct ptf_plan {
   Group g;
@@ -1520,15 +1532,62 @@ int ptf_set_c(ptf_problem* prob, const double* c) {
   return PTF_OK;
 }
 
+// Capture kGraphSteps steps of a primed problem into a graph (kernels only).
+static int build_step_graph(ptf_problem* prob) {
+  prob->drop_graph();
+  Group& g = prob->g;
+  HIPCHK(hipStreamBeginCapture(prob->stream, hipStreamCaptureModeThreadLocal));
+  int rc = PTF_OK;
+  for (int i = 0; i < kGraphSteps && rc == PTF_OK; ++i) {
+    for (int s = 0; s < 4 && rc == PTF_OK; ++s)
+      for (auto& sl : g.stages[s])
+        if ((rc = launch(sl, prob->stream)) != PTF_OK) break;
+  }
+  hipGraph_t graph = nullptr;
+  const hipError_t ce = hipStreamEndCapture(prob->stream, &graph);
+  if (rc != PTF_OK || ce != hipSuccess) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return rc != PTF_OK ? rc : fail(PTF_ERR_DEVICE, std::string("graph capture: ") + hipGetErrorString(ce));
+  }
+  prob->graph = graph;
+  if (hipGraphInstantiate(&prob->exec, prob->graph, nullptr, nullptr, 0) != hipSuccess) {
+    prob->drop_graph();
+    return fail(PTF_ERR_DEVICE, "hipGraphInstantiate failed");
+  }
+  return PTF_OK;
+}
+
 int ptf_step(ptf_problem* prob, int64_t nsteps) {
   if (!prob) return fail(PTF_ERR_INVALID, "problem is NULL");
   if (nsteps < 0) return fail(PTF_ERR_INVALID, "nsteps must be >= 0");
-  for (int64_t i = 0; i < nsteps; ++i) {
-    CHK(group_step_launches(prob->g, prob->stream, prob->t));
-    prob->t += prob->g.dt;
+  Group& g = prob->g;
+  int64_t done = 0;
+  // steady flows: whole blocks of kGraphSteps steps replay a captured graph (the
+  // steps' kernels and arguments do not depend on the clock)
+  if (prob->use_graph && g.flowsyn.empty() && nsteps >= kGraphSteps) {
+    CHK(group_prime(g, prob->stream));
+    if (!prob->exec) CHK(build_step_graph(prob));
+    for (; done + kGraphSteps <= nsteps; done += kGraphSteps) {
+      HIPCHK(hipGraphLaunch(prob->exec, prob->stream));
+      for (int i = 0; i < kGraphSteps; ++i) {
+        prob->t += g.dt;
+        prob->step += 1;
+      }
+    }
+  }
+  for (int64_t i = done; i < nsteps; ++i) {
+    CHK(group_step_launches(g, prob->stream, prob->t));
+    prob->t += g.dt;
     prob->step += 1;
   }
   HIPCHK(hipGetLastError());
+  return PTF_OK;
+}
+
+int ptf_problem_set_graph(ptf_problem* prob, int32_t enable) {
+  if (!prob) return fail(PTF_ERR_INVALID, "problem is NULL");
+  prob->use_graph = enable != 0;
+  prob->drop_graph();
   return PTF_OK;
 }
 
@@ -1782,6 +1841,7 @@ int ptf_problem_sol_ptr(ptf_problem* prob, void** dptr) {
 
 int ptf_autotune(ptf_problem* prob) {
   if (!prob) return fail(PTF_ERR_INVALID, "problem is NULL");
+  prob->drop_graph();
   return autotune_group(prob->g, prob->stream);
 }
 
@@ -1828,6 +1888,7 @@ int ptf_problem_set_variant(ptf_problem* prob, int32_t slot, int32_t R, int32_t 
                                          " W=" + std::to_string(W) + " flags=" +
                                          std::to_string(flags) + " for slot " + trial.name);
   *v[slot] = trial;
+  prob->drop_graph();
   return PTF_OK;
 }
 

@@ -92,6 +92,7 @@ SIGNATURES = {
     "ptf_set_analytic_flow": [_VP, C.POINTER(ptf_flow_term), _I32],
     "ptf_set_c": [_VP, _DP],
     "ptf_step": [_VP, _I64],
+    "ptf_problem_set_graph": [_VP, _I32],
     "ptf_get_clock": [_VP, _DP, C.POINTER(_I64)],
     "ptf_set_clock": [_VP, C.c_double, _I64],
     "ptf_get_sol": [_VP, _DP],

@@ -333,6 +333,10 @@ class Problem:
     def autotune(self):
         call("ptf_autotune", self._h)
 
+    def set_graph(self, enable: bool):
+        """Replay captured 8-step graphs for steady flows (default) or launch directly."""
+        call("ptf_problem_set_graph", self._h, 1 if enable else 0)
+
     def slots(self):
         n = C.c_int32()
         call("ptf_problem_num_slots", self._h, C.byref(n))
