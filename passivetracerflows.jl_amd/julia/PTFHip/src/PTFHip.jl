@@ -52,6 +52,20 @@ struct PtfDesc
   filter_tol::Float64
 end
 
+"One separable term of a built-in analytic flow (ptf_flow_term in include/ptf.h)."
+struct PtfFlowTerm
+  component::Int32            # 0 = u, 1 = v, 2 = w
+  fx::Int32; fy::Int32; fz::Int32   # 0 = 1, 1 = sin, 2 = cos
+  kx::Float64; ky::Float64; kz::Float64
+  px::Float64; py::Float64; pz::Float64
+  amp::Float64
+  time_fn::Int32
+  reserved::Int32
+  omega::Float64
+  tphase::Float64
+end
+const PTF_FLOW_ANALYTIC = Int32(3)
+
 lasterror() = unsafe_string(ccall((:ptf_last_error, libptf), Cstring, ()))
 check(rc) = rc == 0 || error("libptf_hip error $rc: $(lasterror())")
 
@@ -78,6 +92,7 @@ end
 
 mutable struct RFFTPlan
   h::Ptr{Cvoid}
+  RFFTPlan(h::Ptr{Cvoid}) = new(h)
   function RFFTPlan(ndim, n::NTuple{3,Int64}, precision)
     h = Ref{Ptr{Cvoid}}(C_NULL)
     check(ccall((:ptf_plan_create, libptf), Cint, (Int32, Ref{NTuple{3,Int64}}, Int32,
@@ -87,6 +102,21 @@ mutable struct RFFTPlan
     return p
   end
 end
+
+"Layered plan: nlayers fields of the grid transformed over dims 1:ndim (S:678-679, S:865-866)."
+function LayeredRFFTPlan(ndim, n::NTuple{3,Int64}, nlayers::Integer, precision)
+  h = Ref{Ptr{Cvoid}}(C_NULL)
+  check(ccall((:ptf_plan_create_batched, libptf), Cint, (Int32, Ref{NTuple{3,Int64}}, Int64, Int32,
+              Ref{Ptr{Cvoid}}), ndim, n, nlayers, precision, h))
+  p = RFFTPlan(h[])
+  finalizer(x -> ccall((:ptf_plan_destroy, libptf), Cint, (Ptr{Cvoid},), x.h), p)
+  return p
+end
+
+"dealias!(fh, grid) for a device half spectrum (aliased_fraction 1/3 = the 2/3 rule)."
+dealias!(fh::PtfArray, plan::RFFTPlan; aliased_fraction=1/3) =
+  check(ccall((:ptf_dealias, libptf), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Float64), plan.h, fh.ptr,
+              aliased_fraction))
 
 mul!(out::PtfArray, plan::RFFTPlan, a::PtfArray) =
   (check(ccall((:ptf_rfft, libptf), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}),
@@ -149,19 +179,28 @@ TracerAdvectionDiffusion.Problem (src/traceradvectiondiffusion.jl:143-216) on th
 ROCm device.  `T=Float32` selects the mixed-precision form.
 """
 function Problem(dev::ROCm, flow; nx=128, Lx=2π, ny=nx, Ly=Lx, nz=nx, Lz=Lx, κ=0.1, η=κ, ι=κ,
-                 κh=0.0, nκh=0, dt=0.01, stepper="RK4", T=Float64)
+                 κh=0.0, nκh=0, dt=0.01, stepper="RK4", T=Float64,
+                 order=0, innerK=0.65, outerK=1.0, tol=1e-15,
+                 analytic::Union{Nothing,Vector{PtfFlowTerm}}=nothing)
   nd = ndims_of(flow)
   n = (nx, nd >= 2 ? ny : 1, nd >= 3 ? nz : 1)
   L = (Float64(Lx), nd >= 2 ? Float64(Ly) : 1.0, nd >= 3 ? Float64(Lz) : 1.0)
+  # filter keywords of FilteredRK4/FilteredETDRK4 ([FF] makefilter); order 0 = its defaults
   d = PtfDesc(nd, T == Float64 ? PTF_FP64 : PTF_MIXED, Int64.(n), L, κ, η, ι, κh, nκh,
               STEPPERS[stepper], dt,
-              flow.steadyflow ? PTF_FLOW_STEADY : PTF_FLOW_PER_STEP, 0, 0.0, 0.0, 0.0)
+              analytic !== nothing ? PTF_FLOW_ANALYTIC :
+              (flow.steadyflow ? PTF_FLOW_STEADY : PTF_FLOW_PER_STEP), order, innerK, outerK, tol)
   h = Ref{Ptr{Cvoid}}(C_NULL)
   check(ccall((:ptf_problem_create, libptf), Cint, (Ref{PtfDesc}, Ref{Ptr{Cvoid}}), d, h))
   vars = (c = zeros(Float64, n[1:nd]...), ch = zeros(ComplexF64, n[1] ÷ 2 + 1, n[2:nd]...))
   p = DeviceProblem(h[], nd, n, L, flow, flow.steadyflow, Clock(Float64(dt), 0.0, 0), vars)
   finalizer(x -> ccall((:ptf_problem_destroy, libptf), Cint, (Ptr{Cvoid},), x.h), p)
-  p.steadyflow && upload_flow!(p, 0.0)
+  if analytic !== nothing        # `flow` only gives the dimension; the GPU synthesises u, v, w
+    p.steadyflow = true
+    set_analytic_flow!(p, analytic)
+  elseif p.steadyflow
+    upload_flow!(p, 0.0)
+  end
   return p
 end
 
@@ -183,6 +222,34 @@ function sync_clock!(p::DeviceProblem)
   t = Ref{Float64}(0); s = Ref{Int64}(0)
   check(ccall((:ptf_get_clock, libptf), Cint, (Ptr{Cvoid}, Ref{Float64}, Ref{Int64}), p.h, t, s))
   p.clock.t = t[]; p.clock.step = s[]
+end
+
+"""
+    stepforward!(prob, diags, nsteps)
+
+FourierFlows' stepforward! with diagnostics; when every diagnostic is `mass` or
+`variance` (given as symbols :mass, :variance here), they are recorded on the GPU by
+one ptf_step_record call every `freq` steps and returned as (t, steps, values).
+"""
+function step_record!(p::DeviceProblem, nsteps::Integer, freq::Integer)
+  s0 = p.clock.step
+  nrec = div(s0 + nsteps, freq) - div(s0, freq)
+  t = zeros(max(nrec, 1)); st = zeros(Int64, max(nrec, 1))
+  m = zeros(max(nrec, 1)); v = zeros(max(nrec, 1))
+  n = Ref{Int64}(0)
+  check(ccall((:ptf_step_record, libptf), Cint,
+              (Ptr{Cvoid}, Int64, Int64, Int64, Ref{Int64}, Ptr{Float64}, Ptr{Int64}, Ptr{Float64},
+               Ptr{Float64}, Ptr{Float64}),
+              p.h, nsteps, freq, max(nrec, 1), n, t, st, m, v, C_NULL))
+  sync_clock!(p)
+  k = n[]
+  return (t=t[1:k], steps=st[1:k], mass=m[1:k], variance=v[1:k])
+end
+
+"Use a built-in analytic flow (terms synthesised on the GPU at clock.t every step)."
+function set_analytic_flow!(p::DeviceProblem, terms::Vector{PtfFlowTerm})
+  check(ccall((:ptf_set_analytic_flow, libptf), Cint, (Ptr{Cvoid}, Ptr{PtfFlowTerm}, Int32), p.h,
+              terms, length(terms)))
 end
 
 function stepforward!(p::DeviceProblem, nsteps::Integer=1)
