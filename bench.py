@@ -170,6 +170,16 @@ def trace_summary(trace_dir, bench_out, steps, warmup):
     return "\n".join(lines)
 
 
+def TAD_mass(p):
+    import passivetracerflows as ptf
+    return ptf.TracerAdvectionDiffusion.mass(p)
+
+
+def TAD_variance(p):
+    import passivetracerflows as ptf
+    return ptf.TracerAdvectionDiffusion.variance(p)
+
+
 def gps(points, steps, seconds):
     return points * steps / seconds
 
@@ -291,6 +301,16 @@ def ensemble_parity(members, e32, e64, dt, steps):
                 max_relL2_resolved=worst_resolved, members_within_1e5=within)
 
 
+def mixed_errors(TAD, p32, p64, c0):
+    """Mixed form against the FP64 run of the same config after the same steps: relative
+    L2 of sol, mass error relative to sum|c0|, variance error relative to the variance."""
+    s32, s64 = p32.sol, p64.sol
+    v64 = TAD.variance(p64)
+    return dict(relL2_sol=float(np.linalg.norm(s32 - s64) / np.linalg.norm(s64)),
+                mass_err_rel_l1=abs(TAD.mass(p32) - TAD.mass(p64)) / float(np.abs(c0).sum()),
+                variance_err_rel=abs(TAD.variance(p32) - v64) / v64)
+
+
 def extra_forms(dev, steps, ens_steps):
     import passivetracerflows as ptf
     from passivetracerflows import workloads as W
@@ -299,6 +319,7 @@ def extra_forms(dev, steps, ens_steps):
     # 2D cellular 256^2, FP64 and mixed
     cfg = W.cellular_2d(256)
     u, v = cfg["flow"]
+    probs = {}
     for T, tag in ((np.float64, "fp64"), (np.float32, "mixed")):
         p = TAD.Problem(dev, ptf.TwoDAdvectingFlow(u=lambda x, y: u, v=lambda x, y: v), nx=256,
                         κ=cfg["kappa"], dt=cfg["dt"], T=T)
@@ -309,6 +330,9 @@ def extra_forms(dev, steps, ens_steps):
         s = time_steps(lambda n: ptf.stepforward_(p, n), k)
         out[f"cellular2d_256_{tag}"] = dict(value=gps(256 * 256, k, s), ms_per_step=1e3 * s / k,
                                             steps=k)
+        probs[tag] = p
+    out["cellular2d_256_mixed"]["vs_fp64_after_steps"] = dict(
+        steps=10 + max(steps * 4, 200), **mixed_errors(TAD, probs["mixed"], probs["fp64"], cfg["c0"]))
     # 2D 2048^2 random-phase flow, ETDRK4
     cfg = W.turbulent_2d(2048)
     for T, tag in ((np.float64, "fp64"), (np.float32, "mixed")):
@@ -322,6 +346,9 @@ def extra_forms(dev, steps, ens_steps):
         s = time_steps(lambda n: ptf.stepforward_(p, n), k)
         out[f"turbulent2d_2048_etdrk4_{tag}"] = dict(value=gps(2048 * 2048, k, s),
                                                      ms_per_step=1e3 * s / k, steps=k)
+        probs[tag] = p
+    out["turbulent2d_2048_etdrk4_mixed"]["vs_fp64_after_steps"] = dict(
+        steps=3 + max(steps // 2, 20), **mixed_errors(TAD, probs["mixed"], probs["fp64"], cfg["c0"]))
     # 512-member ensemble, FP32 bulk, hipGraph; parity vs the FP64 ensemble on the GPU
     ens_cfg = W.ensemble_2d()
     members = [dict(m, **W.ensemble_member(m["n"], m["seed"])) for m in ens_cfg["members"]]
@@ -420,6 +447,8 @@ def main():
                                 autotune=not args.no_autotune)
     s32 = pm.sol
     mixed_err = float(np.linalg.norm(s32 - s64) / np.linalg.norm(s64))
+    mixed_diag = dict(mass_err_rel_l1=abs(TAD_mass(pm) - m1) / float(np.abs(cfg["c0"]).sum()),
+                      variance_err_rel=abs(TAD_variance(pm) - v1) / v1)
 
     forms = {
         "abc3d_256_fp64": dict(value=value, ms_per_step=1e3 * secs / args.steps, steps=args.steps,
@@ -438,7 +467,7 @@ def main():
             steps=args.steps, includes="ptf_set_flow(u,v,w) + set_c! + steps + updatevars!"),
         "abc3d_256_mixed": dict(value=gps(npts, args.steps, secs32),
                                 ms_per_step=1e3 * secs32 / args.steps,
-                                relL2_vs_fp64_after_steps=mixed_err,
+                                relL2_vs_fp64_after_steps=mixed_err, **mixed_diag,
                                 steps_compared=args.steps + args.warmup),
     }
     if not args.no_extra:
