@@ -193,7 +193,10 @@ int ptf_diagnostics(ptf_problem* prob, double* mass, double* variance);
 int ptf_step_record(ptf_problem* prob, int64_t nsteps, int64_t every, int64_t capacity,
                     int64_t* nrec_out, double* t_out, int64_t* step_out, double* mass_out,
                     double* var_out, double* c_out);
-/* Device pointer of sol (S-field of the problem's precision). */
+/* Device pointer of sol: the S-field in the problem's precision and working
+ * layout (when the last axis has >= 64 points it is stored in bricks of 8
+ * planes, element (kx, y, z) at ((z>>3) ny + y)(nx/2+1) 8 + 8 kx + (z&7); use
+ * ptf_get_sol for the reference layout).  For in-HBM consumers of the state. */
 int ptf_problem_sol_ptr(ptf_problem* prob, void** dptr);
 /* Pick the fastest compiled kernel variant for every launch slot. */
 int ptf_autotune(ptf_problem* prob);
