@@ -264,6 +264,21 @@ __device__ __forceinline__ double linear_op(const GridInfo& g, const OpParams& p
 
 enum { EPI_STORE = 0, EPI_RK4 = 1, EPI_ETDRK4 = 2 };
 
+// Non-temporal access to fields that are touched once per stage and not again until
+// the next stage, long after the caches have turned over (the stage state, the
+// ETDRK4 coefficients, the steady flows): they should not displace the spectral
+// fields that the next kernel reads.
+template <typename T>
+__device__ __forceinline__ cpx<T> ld_nt(const cpx<T>* p) {
+  return {__builtin_nontemporal_load(&p->re), __builtin_nontemporal_load(&p->im)};
+}
+__device__ __forceinline__ double ld_nt(const double* p) { return __builtin_nontemporal_load(p); }
+template <typename T>
+__device__ __forceinline__ void st_nt(cpx<T>* p, cpx<T> v) {
+  __builtin_nontemporal_store(v.re, &p->re);
+  __builtin_nontemporal_store(v.im, &p->im);
+}
+
 template <typename T>
 struct StageArgs {
   int kind;     // EPI_*
@@ -361,17 +376,17 @@ __device__ __forceinline__ Rk4Ops<T> rk4_load(const StageArgs<T>& a, long long u
   const cpx<T> z = {(T)0, (T)0};
   Rk4Ops<T> o = {z, z, z, z};
   if (a.stage == 0) {
-    o.x = (a.sol + ubase)[off];
+    o.x = ld_nt(a.sol + ubase + off);
   } else if (a.stage == 1) {
-    o.s = (a.sol + ubase)[off];
-    o.ac = (a.acc + ubase)[off];
+    o.s = ld_nt(a.sol + ubase + off);
+    o.ac = ld_nt(a.acc + ubase + off);
   } else {
-    o.x = (a.stg + ubase)[off];
-    o.s = (a.sol + ubase)[off];
+    o.x = ld_nt(a.stg + ubase + off);
+    o.s = ld_nt(a.sol + ubase + off);
     // the compensated low part exists only in the mixed (FP32) form
     constexpr bool kMixed = sizeof(T) == 4;
-    if (a.stage == 2 || (kMixed && a.sol_lo)) o.ac = (a.acc + ubase)[off];
-    if (kMixed && a.stage == 3 && a.sol_lo) o.lo = (a.sol_lo + ubase)[off];
+    if (a.stage == 2 || (kMixed && a.sol_lo)) o.ac = ld_nt(a.acc + ubase + off);
+    if (kMixed && a.stage == 3 && a.sol_lo) o.lo = ld_nt(a.sol_lo + ubase + off);
   }
   return o;
 }
@@ -388,20 +403,20 @@ __device__ __forceinline__ cpx<T> rk4_finish(const StageArgs<T>& a, long long ub
   const cpx<T> rhs = {N.re + L * x.re, N.im + L * x.im};
   if (a.stage == 0) {
     const cpx<T> nx = {x.re + h2 * rhs.re, x.im + h2 * rhs.im};
-    if (a.store_x) (a.stg + ubase)[off] = nx;
-    (a.acc + ubase)[off] = rhs;
+    if (a.store_x) st_nt(a.stg + ubase + off, nx);
+    st_nt(a.acc + ubase + off, rhs);
     return nx;
   } else if (a.stage == 1) {
     const cpx<T> nx = {o.s.re + h2 * rhs.re, o.s.im + h2 * rhs.im};
-    (a.stg + ubase)[off] = nx;
-    (a.acc + ubase)[off] = {o.ac.re / (T)6 + rhs.re / (T)3, o.ac.im / (T)6 + rhs.im / (T)3};
+    st_nt(a.stg + ubase + off, nx);
+    st_nt(a.acc + ubase + off, {o.ac.re / (T)6 + rhs.re / (T)3, o.ac.im / (T)6 + rhs.im / (T)3});
     return nx;
   } else if (a.stage == 2) {
     const cpx<T> nx = {o.s.re + dt * rhs.re, o.s.im + dt * rhs.im};
-    (a.stg + ubase)[off] = nx;
+    st_nt(a.stg + ubase + off, nx);
     const cpx<T> tot = {o.ac.re + rhs.re / (T)3, o.ac.im + rhs.im / (T)3};
-    if (sizeof(T) == 4 && a.sol_lo) (a.acc + ubase)[off] = tot;
-    else (a.sol + ubase)[off] = {o.s.re + dt * tot.re, o.s.im + dt * tot.im};
+    if (sizeof(T) == 4 && a.sol_lo) st_nt(a.acc + ubase + off, tot);
+    else st_nt(a.sol + ubase + off, {o.s.re + dt * tot.re, o.s.im + dt * tot.im});
     return nx;
   } else {
     if (sizeof(T) == 8 || !a.sol_lo) {
@@ -411,7 +426,7 @@ __device__ __forceinline__ cpx<T> rk4_finish(const StageArgs<T>& a, long long ub
         const T f = (T)filter_value(a.g, a.f_order, a.f_innerK, a.f_decay, ikx, iky, ikz);
         ns = {f * ns.re, f * ns.im};
       }
-      (a.sol + ubase)[off] = ns;
+      st_nt(a.sol + ubase + off, ns);
       return ns;
     }
     // compensated accumulation: (hi, lo) += dt tot with TwoSum + renormalisation
@@ -428,8 +443,8 @@ __device__ __forceinline__ cpx<T> rk4_finish(const StageArgs<T>& a, long long ub
       lr = (T)(xr - (double)hr);
       li_ = (T)(xi - (double)hi_);
     }
-    (a.sol + ubase)[off] = {hr, hi_};
-    (a.sol_lo + ubase)[off] = {lr, li_};
+    st_nt(a.sol + ubase + off, {hr, hi_});
+    st_nt(a.sol_lo + ubase + off, {lr, li_});
     return {hr, hi_};
   }
 }
@@ -440,11 +455,11 @@ __device__ __forceinline__ cpx<T> etdrk4_update(const StageArgs<T>& a, long long
   // coefficients are FP64 arrays; the update runs in FP64 and rounds once
   const double nre = (double)N.re, nim = (double)N.im;
   auto sol_at = [&](double& re, double& im) {
-    const cpx<T> s = (a.sol + ubase)[off];
+    const cpx<T> s = ld_nt(a.sol + ubase + off);
     re = (double)s.re;
     im = (double)s.im;
     if (a.sol_lo) {
-      const cpx<T> lo = (a.sol_lo + ubase)[off];
+      const cpx<T> lo = ld_nt(a.sol_lo + ubase + off);
       re += (double)lo.re;
       im += (double)lo.im;
     }
@@ -452,43 +467,43 @@ __device__ __forceinline__ cpx<T> etdrk4_update(const StageArgs<T>& a, long long
   if (a.stage == 0 || a.stage == 1) {
     double sr, si;
     sol_at(sr, si);
-    const double e2 = (a.E2 + ubase)[off], z = (a.zeta + ubase)[off];
+    const double e2 = ld_nt(a.E2 + ubase + off), z = ld_nt(a.zeta + ubase + off);
     const cpx<T> v = {(T)(e2 * sr + z * nre), (T)(e2 * si + z * nim)};
     if (a.stage == 0) {
-      (a.stg + ubase)[off] = v;
-      (a.n1 + ubase)[off] = N;
+      st_nt(a.stg + ubase + off, v);
+      st_nt(a.n1 + ubase + off, N);
     } else {
-      (a.stg2 + ubase)[off] = v;
-      (a.n2 + ubase)[off] = N;
+      st_nt(a.stg2 + ubase + off, v);
+      st_nt(a.n2 + ubase + off, N);
     }
     return v;
   } else if (a.stage == 2) {
-    const cpx<T> s1 = (a.stg + ubase)[off];
+    const cpx<T> s1 = ld_nt(a.stg + ubase + off);
     double sr, si;
     sol_at(sr, si);
-    const cpx<T> N1 = (a.n1 + ubase)[off];
-    const cpx<T> N2 = (a.n2 + ubase)[off];
-    const double e2 = (a.E2 + ubase)[off], z = (a.zeta + ubase)[off];
-    const double e = (a.E + ubase)[off], al = (a.alpha + ubase)[off];
-    const double be2 = 2.0 * (a.beta + ubase)[off];
+    const cpx<T> N1 = ld_nt(a.n1 + ubase + off);
+    const cpx<T> N2 = ld_nt(a.n2 + ubase + off);
+    const double e2 = ld_nt(a.E2 + ubase + off), z = ld_nt(a.zeta + ubase + off);
+    const double e = ld_nt(a.E + ubase + off), al = ld_nt(a.alpha + ubase + off);
+    const double be2 = 2.0 * ld_nt(a.beta + ubase + off);
     const cpx<T> v = {(T)(e2 * (double)s1.re + z * (2.0 * nre - (double)N1.re)),
                       (T)(e2 * (double)s1.im + z * (2.0 * nim - (double)N1.im))};
-    (a.stg2 + ubase)[off] = v;
+    st_nt(a.stg2 + ubase + off, v);
     const double ar = e * sr + al * (double)N1.re + be2 * ((double)N2.re + nre);
     const double ai = e * si + al * (double)N1.im + be2 * ((double)N2.im + nim);
     const cpx<T> hi = {(T)ar, (T)ai};
-    (a.acc + ubase)[off] = hi;
-    if (a.sol_lo) (a.n2 + ubase)[off] = {(T)(ar - (double)hi.re), (T)(ai - (double)hi.im)};  // low part
+    st_nt(a.acc + ubase + off, hi);
+    if (a.sol_lo) st_nt(a.n2 + ubase + off, {(T)(ar - (double)hi.re), (T)(ai - (double)hi.im)});  // low part
     return v;
   } else {
-    const cpx<T> ac = (a.acc + ubase)[off];
+    const cpx<T> ac = ld_nt(a.acc + ubase + off);
     double ar = (double)ac.re, ai = (double)ac.im;
     if (a.sol_lo) {
-      const cpx<T> al = (a.n2 + ubase)[off];
+      const cpx<T> al = ld_nt(a.n2 + ubase + off);
       ar += (double)al.re;
       ai += (double)al.im;
     }
-    const double g = (a.gam + ubase)[off];
+    const double g = ld_nt(a.gam + ubase + off);
     double sr = ar + g * nre, si = ai + g * nim;
     if (a.filtered) {
       const double f = filter_value(a.g, a.f_order, a.f_innerK, a.f_decay, ikx, iky, ikz);
@@ -496,8 +511,8 @@ __device__ __forceinline__ cpx<T> etdrk4_update(const StageArgs<T>& a, long long
       si *= f;
     }
     const cpx<T> hi = {(T)sr, (T)si};
-    (a.sol + ubase)[off] = hi;
-    if (a.sol_lo) (a.sol_lo + ubase)[off] = {(T)(sr - (double)hi.re), (T)(si - (double)hi.im)};
+    st_nt(a.sol + ubase + off, hi);
+    if (a.sol_lo) st_nt(a.sol_lo + ubase + off, {(T)(sr - (double)hi.re), (T)(si - (double)hi.im)});
     return hi;
   }
 }

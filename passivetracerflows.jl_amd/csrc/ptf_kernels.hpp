@@ -380,14 +380,15 @@ __device__ __forceinline__ void stage_row_pairs(const cpx<T> (&x)[R], cpx<T> las
   for (int q = 0; q < R; ++q) xm[q] = lds[lay.idx(c, M - (j + q * P))];
 }
 
-// Row element k is row[k << lb] (brick layout; lb = 0 for contiguous rows).
+// Row element k is row[k << lb] (brick layout; lb = 0 for contiguous rows).  The x
+// pass reads each spectral row once per stage: non-temporal loads.
 template <typename T, int M, int R>
 __device__ __forceinline__ void load_row(const cpx<T>* __restrict__ row, cpx<T> (&x)[R],
                                          cpx<T>& last, int j, int lb) {
   constexpr int P = M / R;
 #pragma unroll
-  for (int q = 0; q < R; ++q) x[q] = row[(uint32_t)(j + q * P) << lb];
-  if (j == 0) last = row[(uint32_t)M << lb];
+  for (int q = 0; q < R; ++q) x[q] = ld_nt(row + ((uint32_t)(j + q * P) << lb));
+  if (j == 0) last = ld_nt(row + ((uint32_t)M << lb));
 }
 
 template <typename T, int M, int R, class Lay>
@@ -487,7 +488,7 @@ void row_kernel(RowArgs<T> a) {
         cpx<T> fv[kPre ? R : 1];
         if constexpr (kPre) {
 #pragma unroll
-          for (int q = 0; q < R; ++q) fv[q] = fl[po + ((uint32_t)(j + q * P) << lb)];
+          for (int q = 0; q < R; ++q) fv[q] = ld_nt(fl + po + ((uint32_t)(j + q * P) << lb));
         }
 #pragma unroll
         for (int q = 0; q < R; ++q) {
@@ -505,7 +506,7 @@ void row_kernel(RowArgs<T> a) {
         for (int q = 0; q < R; ++q) {
           cpx<T> uu;
           if constexpr (kPre) uu = fv[q];
-          else uu = fl[po + ((uint32_t)(j + q * P) << lb)];
+          else uu = ld_nt(fl + po + ((uint32_t)(j + q * P) << lb));
           if (d == 0) {
             f[q] = {-uu.re * z[q].re, -uu.im * z[q].im};
           } else {
@@ -612,7 +613,7 @@ void row_adv_par_kernel(RowArgs<T> a) {
   const cpx<T>* fl = reinterpret_cast<const cpx<T>*>(flp + pub);
   cpx<T> fv[R];
 #pragma unroll
-  for (int q = 0; q < R; ++q) fv[q] = fl[po + ((uint32_t)(j + q * P) << lb)];
+  for (int q = 0; q < R; ++q) fv[q] = ld_nt(fl + po + ((uint32_t)(j + q * P) << lb));
   cpx<T> zm[R];
   stage_row_pairs<T, M, R, Lay>(z, last, zm, lds, lay, c, j);
 #pragma unroll
