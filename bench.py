@@ -18,6 +18,7 @@ restatement (scipy pocketfft threads + numpy) on a bounded number of steps.
 from __future__ import annotations
 
 import argparse
+import faulthandler
 import json
 import os
 import sys
@@ -185,8 +186,10 @@ def gps(points, steps, seconds):
 
 
 def sync():
-    import torch
-    torch.cuda.synchronize()
+    """hipDeviceSynchronize through the library, so the timed process binds only the HIP
+    runtime the library was built against (importing torch would bind torch's own copy)."""
+    from passivetracerflows._lib import call
+    call("ptf_device_synchronize")
 
 
 def time_steps(stepfn, steps):
@@ -396,6 +399,7 @@ def main():
                     help="only run a plain device copy of 1 GiB five times (the reference "
                          "kernel for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE) and exit")
     args = ap.parse_args()
+    faulthandler.enable()
     if args.pmc_summary:
         print(json.dumps(pmc_summary(*args.pmc_summary), indent=1))
         return
@@ -407,8 +411,8 @@ def main():
               "and has no multi-GPU path; use --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
-    import torch  # noqa: F401  (device plumbing: synchronize)
     if args.copy_calibration:
+        import torch          # the runtime's plain device copy, as the HBM-counter yardstick
         n = 1 << 27                              # 2^27 doubles = 1 GiB
         a = torch.ones(n, dtype=torch.float64, device="cuda")
         b = torch.empty_like(a)

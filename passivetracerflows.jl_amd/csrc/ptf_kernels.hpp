@@ -160,7 +160,13 @@ void col_kernel(ColArgs<T> a) {
   } else {
     const cpx<T>* in0 = a.in0 + ubase;
 #pragma unroll
-    for (int q = 0; q < R; ++q) v[q] = in0[off(q)];
+    for (int q = 0; q < R; ++q) {
+      // the 3D y-inverse pass reads its inputs once (unless it re-reads in0 below for
+      // want of registers): non-temporal loads there
+      constexpr bool kNt = MODE == COL_D2 && (sizeof(T) == 4 || R <= 8);
+      if constexpr (kNt) v[q] = ld_nt(in0 + off(q));
+      else v[q] = in0[off(q)];
+    }
 
     if constexpr (MODE == COL_PLAIN) {
       line_fft<T, N, R, DIR, Lay>(v, lds, lay, c, j, a.tw, a.twstride);
@@ -182,7 +188,7 @@ void col_kernel(ColArgs<T> a) {
         if constexpr (MODE == COL_D2) {
           const cpx<T>* in1 = a.in1 + ubase;
 #pragma unroll
-          for (int q = 0; q < R; ++q) p2[q] = in1[off(q)];
+          for (int q = 0; q < R; ++q) p2[q] = ld_nt(in1 + off(q));
         }
       }
 #pragma unroll
@@ -216,7 +222,7 @@ void col_kernel(ColArgs<T> a) {
 #pragma unroll
         for (int q = 0; q < R; ++q) {
           if constexpr (kKeep) v[q] = p2[q];
-          else v[q] = in1[off(q)];
+          else v[q] = ld_nt(in1 + off(q));
         }
         line_fft<T, N, R, +1, Lay>(v, lds, lay, c, j, a.tw, a.twstride);
         cpx<T>* o = a.out2 + ubase;
