@@ -93,8 +93,8 @@ static const char* kind_name(int kind) {
 }
 
 static const int kRadices[] = {16, 8, 4};  // tried in this order
-static const int kColWidths[] = {16, 8, 32, 64, 4, 2};
-static const int kNumColWidths = 6;
+static const int kColWidths[] = {16, 8, 32, 64, 4, 2, 1};
+static const int kNumColWidths = 7;
 static const int kRowWidths[] = {8, 16, 4, 2, 1, 1};
 static const int kNumRowWidths = 5;
 
@@ -140,9 +140,11 @@ static bool choose_variant(Slot& s, int R, int W, int occ = 0, int par = 0) {
   if (occ && !s.has_occ()) return false;
   if (par && !s.has_par()) return false;
   if (s.brick > 0) {
-    // a bricked line needs whole bricks per thread group; bricked rows whole bricks per block
+    // a bricked line needs whole bricks per thread group; bricked rows go in whole bricks
+    // or in power-of-two parts of one brick per block (every row width divides or is a
+    // multiple of the brick)
     if (s.is_col() && (R > s.N || (s.N / R) % (1 << s.brick) != 0)) return false;
-    if (!s.is_col() && W % (1 << s.brick) != 0) return false;
+    if (!s.is_col() && W % (1 << s.brick) != 0 && (1 << s.brick) % W != 0) return false;
   }
   const void* p = f(s.N, R, W + 100 * par + 1000 * occ);
   if (!p) return false;

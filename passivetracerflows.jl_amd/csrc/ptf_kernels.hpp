@@ -406,6 +406,16 @@ __device__ __forceinline__ void load_row_pairs(const cpx<T>* __restrict__ row, c
   stage_row_pairs<T, M, R, Lay>(x, last, xm, lds, lay, c, j);
 }
 
+constexpr int ilog2(int n) { return n <= 1 ? 0 : 1 + ilog2(n / 2); }
+
+// Row r of a bricked array starts at ((r >> lb) * len << lb) + (r mod 2^lb), len = the
+// row length in elements (nx/2 + 1 spectral values, or nx/2 physical x-pairs).  A block
+// of NL rows starting at row0 holds whole bricks (NL >= 2^lb) or NL consecutive rows of
+// one brick (NL < 2^lb).
+__device__ __forceinline__ long long brick_row_base(long long row0, int lb, int len) {
+  return (((row0 >> lb) * len) << lb) + (row0 & ((1LL << lb) - 1));
+}
+
 template <typename T, int NX, int R, int NL, int MODE, int NIN, int OCC = 0>
 __global__ __attribute__((amdgpu_flat_work_group_size(1, NL * ((NX / 2) / R)),
                           amdgpu_waves_per_eu(
@@ -423,21 +433,29 @@ void row_kernel(RowArgs<T> a) {
   const Lay lay = a.lb ? Lay{1, NL} : Lay{M + PAD, 1};
   __shared__ cpx<T> lds[NL * (M + PAD)];
   const int t = threadIdx.x;
-  // lane t = rb + 2^lb (j + P g): the rows of a brick vary fastest (NL % 2^lb == 0)
+  // lane t = rb + 2^lt (j + P g), 2^lt = min(2^lb, NL): the block's rows of a brick vary
+  // fastest.  With part of a brick per block (NL < 2^lb) the sibling blocks of a brick,
+  // which share its cache lines, are renumbered onto one XCD.
   const int lb = a.lb;
-  const int rb = t & ((1 << lb) - 1);
-  const int rest = t >> lb;
+  const int lt = lb < ilog2(NL) ? lb : ilog2(NL);
+  int bx = blockIdx.x, member = blockIdx.y;
+  if (lt < lb) {
+    int bz;
+    xcd_remap(bx, member, bz);
+  }
+  const int rb = t & ((1 << lt) - 1);
+  const int rest = t >> lt;
   const int j = rest % P;
   const int g = rest / P;
-  const int c = (g << lb) + rb;                                  // row within the block
-  const long long row = (long long)blockIdx.x * NL + c;
+  const int c = (g << lt) + rb;                                  // row within the block
+  const long long row0 = (long long)bx * NL;
+  const long long row = row0 + c;
   const bool valid = row < a.nrows;
-  const int gmax = (int)((a.nrows - (long long)blockIdx.x * NL) >> lb) - 1;
+  const int gmax = (int)((a.nrows - row0) >> lt) - 1;
   const int gc = valid ? g : (g < gmax ? g : gmax);              // clamped brick row
-  const int member = blockIdx.y;
   // block-uniform bases + 32-bit lane offsets
-  const long long sub = (long long)member * a.smstride + (long long)blockIdx.x * NL * NXH;
-  const long long pub = (long long)member * a.pmstride + (long long)blockIdx.x * NL * NX;
+  const long long sub = (long long)member * a.smstride + brick_row_base(row0, lb, NXH);
+  const long long pub = (long long)member * a.pmstride + 2 * brick_row_base(row0, lb, M);
   const uint32_t so = ((uint32_t)gc * NXH << lb) + rb;           // spectral element 0 (loads)
   const uint32_t so_st = ((uint32_t)g * NXH << lb) + rb;         // (stores)
   const uint32_t po = ((uint32_t)gc * M << lb) + rb;             // physical pair 0 (loads)
@@ -594,19 +612,26 @@ void row_adv_par_kernel(RowArgs<T> a) {
   const int grp = (int)threadIdx.x / G;   // uniform over a wavefront
   const int t = (int)threadIdx.x - grp * G;
   cpx<T>* lds = lds_all + grp * (NL * (M + 1));
+  // lane and block mapping as in row_kernel
   const int lb = a.lb;
-  const int rb = t & ((1 << lb) - 1);
-  const int rest = t >> lb;
+  const int lt = lb < ilog2(NL) ? lb : ilog2(NL);
+  int bx = blockIdx.x, member = blockIdx.y;
+  if (lt < lb) {
+    int bz;
+    xcd_remap(bx, member, bz);
+  }
+  const int rb = t & ((1 << lt) - 1);
+  const int rest = t >> lt;
   const int j = rest % P;
   const int g = rest / P;
-  const int c = (g << lb) + rb;
-  const long long row = (long long)blockIdx.x * NL + c;
+  const int c = (g << lt) + rb;
+  const long long row0 = (long long)bx * NL;
+  const long long row = row0 + c;
   const bool valid = row < a.nrows;
-  const int gmax = (int)((a.nrows - (long long)blockIdx.x * NL) >> lb) - 1;
+  const int gmax = (int)((a.nrows - row0) >> lt) - 1;
   const int gc = valid ? g : (g < gmax ? g : gmax);
-  const int member = blockIdx.y;
-  const long long sub = (long long)member * a.smstride + (long long)blockIdx.x * NL * NXH;
-  const long long pub = (long long)member * a.pmstride + (long long)blockIdx.x * NL * NX;
+  const long long sub = (long long)member * a.smstride + brick_row_base(row0, lb, NXH);
+  const long long pub = (long long)member * a.pmstride + 2 * brick_row_base(row0, lb, M);
   const uint32_t so = ((uint32_t)gc * NXH << lb) + rb;
   const uint32_t so_st = ((uint32_t)g * NXH << lb) + rb;
   const uint32_t po = ((uint32_t)gc * M << lb) + rb;

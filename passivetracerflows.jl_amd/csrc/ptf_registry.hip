@@ -104,6 +104,9 @@ template <typename T, int N, int R, int KIND, int OCC>
 const void* by_w_occ(int W) {
   if constexpr (KIND < 10) {
     switch (W) {
+      case 1:   // one line per block: long lines only (N >= 512), where LDS caps the blocks per CU
+        if constexpr (N >= 512) return colp<T, N, R, 1, KIND, OCC>();
+        return nullptr;
       case 2: return colp<T, N, R, 2, KIND, OCC>();
       case 4: return colp<T, N, R, 4, KIND, OCC>();
       case 8: return colp<T, N, R, 8, KIND, OCC>();

@@ -53,6 +53,9 @@ CASES = [
     (3, (32, 32, 64), "ETDRK4", np.float64),
     (2, (128, 128), "RK4", np.float64), (2, (256, 64), "ETDRK4", np.float32),
     (1, (256,), "RK4", np.float64),
+    # long lines (>= 512): one-line-per-block column variants and part-brick row blocks
+    (2, (64, 512), "RK4", np.float64), (2, (128, 512), "ETDRK4", np.float32),
+    (3, (16, 512, 64), "RK4", np.float32), (3, (16, 16, 512), "ETDRK4", np.float64),
 ]
 
 
@@ -60,7 +63,8 @@ CASES = [
 def test_every_variant_of_every_slot(dev, ndim, n, stepper, T):
     print(f"variants ndim={ndim} n={n} {stepper} {T.__name__}", flush=True)
     g, flows, c0 = _setup(ndim, n, seed=11 + ndim)
-    dt, kappa, nsteps = 0.01, 1e-2, 3
+    # kappa |k_max|^2 dt stays inside RK4's real-axis stability limit for lines of 512
+    dt, kappa, nsteps = 0.01, (1e-3 if max(n) >= 512 else 1e-2), 3
     o = O.Problem(g, flows, kappa=kappa, dt=dt, stepper=stepper)
     o.set_c(c0)
     sol0 = o.sol.copy()
