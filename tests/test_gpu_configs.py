@@ -50,16 +50,19 @@ def test_cellular_2d_full(dev, T, tol):
     cfg = W.cellular_2d(256)
     print(f"cellular 256^2 x {cfg['nsteps']} {T.__name__}", flush=True)
     u, v = cfg["flow"]
-    p = TAD.Problem(dev, ptf.TwoDAdvectingFlow(*_fl((u, v))), nx=256, κ=cfg["kappa"],
-                    dt=cfg["dt"], T=T)
-    TAD.set_c_(p, cfg["c0"])
-    m0 = TAD.mass(p)
-    ptf.stepforward_(p, cfg["nsteps"])
     o = O.Problem(O.Grid(2, 256), (u, v), kappa=cfg["kappa"], dt=cfg["dt"])
     o.set_c(cfg["c0"])
     o.stepforward(cfg["nsteps"])
-    assert rel(p.sol, o.sol) <= tol
-    assert abs(TAD.mass(p) - m0) <= 1e-9 * np.abs(cfg["c0"]).sum()
+    for tune in (False, True):   # default variants, then the autotuned (benchmarked) ones
+        p = TAD.Problem(dev, ptf.TwoDAdvectingFlow(*_fl((u, v))), nx=256, κ=cfg["kappa"],
+                        dt=cfg["dt"], T=T)
+        TAD.set_c_(p, cfg["c0"])
+        if tune:
+            p.autotune()
+        m0 = TAD.mass(p)
+        ptf.stepforward_(p, cfg["nsteps"])
+        assert rel(p.sol, o.sol) <= tol, tune
+        assert abs(TAD.mass(p) - m0) <= 1e-9 * np.abs(cfg["c0"]).sum()
 
 
 def test_abc_3d_short_parity_and_full_properties(dev):
@@ -75,6 +78,13 @@ def test_abc_3d_short_parity_and_full_properties(dev):
     o.set_c(cfg["c0"])
     o.stepforward(3)
     assert rel(p.sol, o.sol) <= 1e-12
+    # the variants the autotuner picks at this size (the benchmarked ones)
+    pt = mk(np.float64)
+    TAD.set_c_(pt, cfg["c0"])
+    pt.autotune()
+    ptf.stepforward_(pt, 3)
+    assert rel(pt.sol, o.sol) <= 1e-12, [pt.get_variant(i) for i in range(len(pt.slots()))]
+    del pt
     # full run: properties
     l1 = np.abs(cfg["c0"]).sum()
     m0, v0 = TAD.mass(p), TAD.variance(p)
@@ -105,13 +115,17 @@ def test_turbulent_2d_short_parity(dev):
                   nkappah=cfg["nkappah"], dt=cfg["dt"], stepper="ETDRK4")
     o.set_c(cfg["c0"])
     o.stepforward(nsteps)
-    for T, tol in ((np.float64, 1e-12), (np.float32, 1e-5)):
+    # default variants, then the autotuned ones (part-brick rows and one-line columns)
+    for T, tol, tune in ((np.float64, 1e-12, False), (np.float32, 1e-5, False),
+                         (np.float64, 1e-12, True), (np.float32, 1e-5, True)):
         p = TAD.Problem(dev, ptf.SpectralStreamfunctionFlow(cfg["psih0"], cfg["omega"]), nx=n,
                         κ=0.0, η=0.0, κh=cfg["kappah"], nκh=cfg["nkappah"], dt=cfg["dt"],
                         stepper="ETDRK4", T=T)
         TAD.set_c_(p, cfg["c0"])
+        if tune:
+            p.autotune()
         ptf.stepforward_(p, nsteps)
-        assert rel(p.sol, o.sol) <= tol, T
+        assert rel(p.sol, o.sol) <= tol, (T, tune, p.slots())
 
 
 def test_turbulent_2d_full_properties(dev):
@@ -142,6 +156,8 @@ def test_ensemble_512_full(dev):
         for i, m in enumerate(members):
             e.set_flow(i, *m["flow"])
             e.set_c(i, m["c0"])
+        if prec == "mixed":
+            e.autotune()   # the benchmarked variants
         e.step(cfg["nsteps"], graph=True)
         ens[prec] = e
     counts = {"resolved": 0, "under_resolved": 0, "diffusion_unstable": 0}
