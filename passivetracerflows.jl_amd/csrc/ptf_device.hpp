@@ -370,14 +370,19 @@ struct Rk4Ops {
   cpx<T> x, s, ac, lo;
 };
 
-template <typename T>
+// The RK4 stage operands and update.  ST >= 0 fixes the stage at compile time (callers
+// that dispatch on the uniform stage once, so that the loads of several elements form
+// one straight-line block); ST = -1 reads it from the arguments.  store = false (lanes
+// of a tail column that only computes) suppresses the state stores.
+template <typename T, int ST = -1>
 __device__ __forceinline__ Rk4Ops<T> rk4_load(const StageArgs<T>& a, long long ubase, uint32_t off) {
+  const int stage = ST >= 0 ? ST : a.stage;
   // every field is defined on every path so that the value stays in registers
   const cpx<T> z = {(T)0, (T)0};
   Rk4Ops<T> o = {z, z, z, z};
-  if (a.stage == 0) {
+  if (stage == 0) {
     o.x = ld_nt(a.sol + ubase + off);
-  } else if (a.stage == 1) {
+  } else if (stage == 1) {
     o.s = ld_nt(a.sol + ubase + off);
     o.ac = ld_nt(a.acc + ubase + off);
   } else {
@@ -385,38 +390,47 @@ __device__ __forceinline__ Rk4Ops<T> rk4_load(const StageArgs<T>& a, long long u
     o.s = ld_nt(a.sol + ubase + off);
     // the compensated low part exists only in the mixed (FP32) form
     constexpr bool kMixed = sizeof(T) == 4;
-    if (a.stage == 2 || (kMixed && a.sol_lo)) o.ac = ld_nt(a.acc + ubase + off);
-    if (kMixed && a.stage == 3 && a.sol_lo) o.lo = ld_nt(a.sol_lo + ubase + off);
+    if (stage == 2 || (kMixed && a.sol_lo)) o.ac = ld_nt(a.acc + ubase + off);
+    if (kMixed && stage == 3 && a.sol_lo) o.lo = ld_nt(a.sol_lo + ubase + off);
   }
   return o;
 }
 
-template <typename T>
+// op = the member's operator coefficients (a.op[member]), read once by the caller: a
+// read through a.op between the state stores would be repeated for every element.
+template <typename T, int ST = -1>
 __device__ __forceinline__ cpx<T> rk4_finish(const StageArgs<T>& a, long long ubase, uint32_t off,
-                                             int member, int ikx, int iky, int ikz, cpx<T> N,
-                                             const Rk4Ops<T>& o) {
-  const T L = (T)linear_op(a.g, a.op[member], a.nkappah, ikx, iky, ikz);
+                                             const OpParams& op, int ikx, int iky, int ikz, cpx<T> N,
+                                             const Rk4Ops<T>& o, bool store = true) {
+  const int stage = ST >= 0 ? ST : a.stage;
+  const T L = (T)linear_op(a.g, op, a.nkappah, ikx, iky, ikz);
   const T dt = (T)a.dt;
   const T h2 = (T)(a.dt / 2);
   // stage 2 input x_2 = sol + dt/2 RHS_1, recomputed exactly as stage 1 formed it
-  const cpx<T> x = a.stage == 1 ? cpx<T>{o.s.re + h2 * o.ac.re, o.s.im + h2 * o.ac.im} : o.x;
+  const cpx<T> x = stage == 1 ? cpx<T>{o.s.re + h2 * o.ac.re, o.s.im + h2 * o.ac.im} : o.x;
   const cpx<T> rhs = {N.re + L * x.re, N.im + L * x.im};
-  if (a.stage == 0) {
+  if (stage == 0) {
     const cpx<T> nx = {x.re + h2 * rhs.re, x.im + h2 * rhs.im};
-    if (a.store_x) st_nt(a.stg + ubase + off, nx);
-    st_nt(a.acc + ubase + off, rhs);
+    if (store) {
+      if (a.store_x) st_nt(a.stg + ubase + off, nx);
+      st_nt(a.acc + ubase + off, rhs);
+    }
     return nx;
-  } else if (a.stage == 1) {
+  } else if (stage == 1) {
     const cpx<T> nx = {o.s.re + h2 * rhs.re, o.s.im + h2 * rhs.im};
-    st_nt(a.stg + ubase + off, nx);
-    st_nt(a.acc + ubase + off, {o.ac.re / (T)6 + rhs.re / (T)3, o.ac.im / (T)6 + rhs.im / (T)3});
+    if (store) {
+      st_nt(a.stg + ubase + off, nx);
+      st_nt(a.acc + ubase + off, {o.ac.re / (T)6 + rhs.re / (T)3, o.ac.im / (T)6 + rhs.im / (T)3});
+    }
     return nx;
-  } else if (a.stage == 2) {
+  } else if (stage == 2) {
     const cpx<T> nx = {o.s.re + dt * rhs.re, o.s.im + dt * rhs.im};
-    st_nt(a.stg + ubase + off, nx);
     const cpx<T> tot = {o.ac.re + rhs.re / (T)3, o.ac.im + rhs.im / (T)3};
-    if (sizeof(T) == 4 && a.sol_lo) st_nt(a.acc + ubase + off, tot);
-    else st_nt(a.sol + ubase + off, {o.s.re + dt * tot.re, o.s.im + dt * tot.im});
+    if (store) {
+      st_nt(a.stg + ubase + off, nx);
+      if (sizeof(T) == 4 && a.sol_lo) st_nt(a.acc + ubase + off, tot);
+      else st_nt(a.sol + ubase + off, {o.s.re + dt * tot.re, o.s.im + dt * tot.im});
+    }
     return nx;
   } else {
     if (sizeof(T) == 8 || !a.sol_lo) {
@@ -426,7 +440,7 @@ __device__ __forceinline__ cpx<T> rk4_finish(const StageArgs<T>& a, long long ub
         const T f = (T)filter_value(a.g, a.f_order, a.f_innerK, a.f_decay, ikx, iky, ikz);
         ns = {f * ns.re, f * ns.im};
       }
-      st_nt(a.sol + ubase + off, ns);
+      if (store) st_nt(a.sol + ubase + off, ns);
       return ns;
     }
     // compensated accumulation: (hi, lo) += dt tot with TwoSum + renormalisation
@@ -443,8 +457,10 @@ __device__ __forceinline__ cpx<T> rk4_finish(const StageArgs<T>& a, long long ub
       lr = (T)(xr - (double)hr);
       li_ = (T)(xi - (double)hi_);
     }
-    st_nt(a.sol + ubase + off, {hr, hi_});
-    st_nt(a.sol_lo + ubase + off, {lr, li_});
+    if (store) {
+      st_nt(a.sol + ubase + off, {hr, hi_});
+      st_nt(a.sol_lo + ubase + off, {lr, li_});
+    }
     return {hr, hi_};
   }
 }
@@ -525,7 +541,7 @@ __device__ __forceinline__ cpx<T> stage_epilogue(const StageArgs<T>& a, long lon
                                                  int ikz, cpx<T> N) {
   if (a.kind == EPI_RK4) {
     const Rk4Ops<T> o = rk4_load(a, ubase, off);
-    return rk4_finish(a, ubase, off, member, ikx, iky, ikz, N, o);
+    return rk4_finish(a, ubase, off, a.op[member], ikx, iky, ikz, N, o);
   }
   return etdrk4_update(a, ubase, off, ikx, iky, ikz, N);
 }

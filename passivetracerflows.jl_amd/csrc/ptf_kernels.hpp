@@ -11,6 +11,8 @@
 // M = nx/2 points plus the real<->half-complex pre/post twiddle; a workgroup
 // owns NL rows, thread (c, j) = (t / P, t % P) so lanes walk along a row.
 #pragma once
+#include <type_traits>
+
 #include "ptf_device.hpp"
 
 namespace ptf {
@@ -236,36 +238,70 @@ void col_kernel(ColArgs<T> a) {
       // transform so that their latency overlaps it (FP64 radix 8/16 would spill)
       constexpr bool kPre = sizeof(T) == 4 || R <= 4;
       Rk4Ops<T> ops[kPre ? R : 1];
-      const bool pre = kPre && a.st.kind == EPI_RK4;
+      const bool rk4 = a.st.kind == EPI_RK4;
+      // The RK4 stage is uniform: dispatch on it once, so that the operand loads of a
+      // chunk of elements form one straight-line block and go out together (one memory
+      // round trip per chunk, not one per element).  FP64 radix >= 8 loads in two chunks
+      // (registers).
+      auto rk4_dispatch = [&](auto body) {
+        switch (a.st.stage) {
+          case 0: body(std::integral_constant<int, 0>{}); break;
+          case 1: body(std::integral_constant<int, 1>{}); break;
+          case 2: body(std::integral_constant<int, 2>{}); break;
+          default: body(std::integral_constant<int, 3>{}); break;
+        }
+      };
       if constexpr (kPre) {
-        if (pre) {
+        if (rk4) {
+          rk4_dispatch([&](auto stc) {
+            constexpr int ST = decltype(stc)::value;
 #pragma unroll
-          for (int q = 0; q < R; ++q) ops[q] = rk4_load(a.st, ubase, off(q));
+            for (int q = 0; q < R; ++q) ops[q] = rk4_load<T, ST>(a.st, ubase, off(q));
+          });
         }
       }
       line_fft<T, N, R, -1, Lay>(v, lds, lay, c, j, a.tw, a.twstride);
       // invalid (tail) lanes compute on a clamped column and store nothing
+      if (rk4) {
+        const OpParams opm = a.st.op[member];
+        rk4_dispatch([&](auto stc) {
+          constexpr int ST = decltype(stc)::value;
+          constexpr int kChunk = (!kPre && sizeof(T) == 8 && R >= 8) ? R / 2 : R;
 #pragma unroll
-      for (int q = 0; q < R; ++q) {
-        const int ia = j + q * P;
-        const int iky = a.axis == 2 ? by : ia;
-        const int ikz = a.axis == 2 ? ia : iklast_col;
-        if constexpr (MODE == COL_STAGE) {
-          if (!valid) continue;
-          if (a.st.kind == EPI_STORE) {
-            (a.out0 + ubase)[off(q)] = v[q];
-            continue;
+          for (int q0 = 0; q0 < R; q0 += kChunk) {
+            Rk4Ops<T> o[kChunk];
+            if constexpr (!kPre) {
+#pragma unroll
+              for (int i = 0; i < kChunk; ++i) o[i] = rk4_load<T, ST>(a.st, ubase, off(q0 + i));
+            }
+#pragma unroll
+            for (int i = 0; i < kChunk; ++i) {
+              const int q = q0 + i;
+              const int ia = j + q * P;
+              const int iky = a.axis == 2 ? by : ia;
+              const int ikz = a.axis == 2 ? ia : iklast_col;
+              Rk4Ops<T> oq;
+              if constexpr (kPre) oq = ops[q];
+              else oq = o[i];
+              v[q] = rk4_finish<T, ST>(a.st, ubase, off(q), opm, ikx_col, iky, ikz, v[q], oq, valid);
+            }
           }
+        });
+      } else {
+#pragma unroll
+        for (int q = 0; q < R; ++q) {
+          const int ia = j + q * P;
+          const int iky = a.axis == 2 ? by : ia;
+          const int ikz = a.axis == 2 ? ia : iklast_col;
+          if constexpr (MODE == COL_STAGE) {
+            if (!valid) continue;
+            if (a.st.kind == EPI_STORE) {
+              (a.out0 + ubase)[off(q)] = v[q];
+              continue;
+            }
+          }
+          if (valid) v[q] = stage_epilogue(a.st, ubase, off(q), member, ikx_col, iky, ikz, v[q]);
         }
-        cpx<T> nx;
-        if (!valid) {
-          nx = v[q];
-        } else if (pre) {
-          if constexpr (kPre) nx = rk4_finish(a.st, ubase, off(q), member, ikx_col, iky, ikz, v[q], ops[q]);
-        } else {
-          nx = stage_epilogue(a.st, ubase, off(q), member, ikx_col, iky, ikz, v[q]);
-        }
-        v[q] = nx;
       }
       if constexpr (MODE == COL_STAGE_D1) {
         // v holds the next stage's input along the line: transform it as COL_D1
