@@ -465,72 +465,110 @@ __device__ __forceinline__ cpx<T> rk4_finish(const StageArgs<T>& a, long long ub
   }
 }
 
+// The ETDRK4 stage operands and update (coefficients are FP64 arrays; the update runs in
+// FP64 and rounds once).  ST and store as for rk4_load / rk4_finish.
 template <typename T>
-__device__ __forceinline__ cpx<T> etdrk4_update(const StageArgs<T>& a, long long ubase,
-                                                uint32_t off, int ikx, int iky, int ikz, cpx<T> N) {
-  // coefficients are FP64 arrays; the update runs in FP64 and rounds once
-  const double nre = (double)N.re, nim = (double)N.im;
-  auto sol_at = [&](double& re, double& im) {
-    const cpx<T> s = ld_nt(a.sol + ubase + off);
-    re = (double)s.re;
-    im = (double)s.im;
-    if (a.sol_lo) {
+struct EtdOps {
+  double sr, si;          // sol (+ its low part) for stages 0-2; acc (+ low part) for stage 3
+  cpx<T> s1, n1, n2;      // stage 2
+  double e2, z, e, al, be, g;
+};
+
+template <typename T, int ST = -1>
+__device__ __forceinline__ EtdOps<T> etd_load(const StageArgs<T>& a, long long ubase, uint32_t off) {
+  const int stage = ST >= 0 ? ST : a.stage;
+  const cpx<T> zc = {(T)0, (T)0};
+  EtdOps<T> o = {0.0, 0.0, zc, zc, zc, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  // the compensated low parts exist only in the mixed (FP32) form
+  constexpr bool kMixed = sizeof(T) == 4;
+  if (stage <= 2) {
+    const cpx<T> s0 = ld_nt(a.sol + ubase + off);
+    o.sr = (double)s0.re;
+    o.si = (double)s0.im;
+    if (kMixed && a.sol_lo) {
       const cpx<T> lo = ld_nt(a.sol_lo + ubase + off);
-      re += (double)lo.re;
-      im += (double)lo.im;
+      o.sr += (double)lo.re;
+      o.si += (double)lo.im;
     }
-  };
-  if (a.stage == 0 || a.stage == 1) {
-    double sr, si;
-    sol_at(sr, si);
-    const double e2 = ld_nt(a.E2 + ubase + off), z = ld_nt(a.zeta + ubase + off);
-    const cpx<T> v = {(T)(e2 * sr + z * nre), (T)(e2 * si + z * nim)};
-    if (a.stage == 0) {
-      st_nt(a.stg + ubase + off, v);
-      st_nt(a.n1 + ubase + off, N);
-    } else {
-      st_nt(a.stg2 + ubase + off, v);
-      st_nt(a.n2 + ubase + off, N);
+    o.e2 = ld_nt(a.E2 + ubase + off);
+    o.z = ld_nt(a.zeta + ubase + off);
+    if (stage == 2) {
+      o.s1 = ld_nt(a.stg + ubase + off);
+      o.n1 = ld_nt(a.n1 + ubase + off);
+      o.n2 = ld_nt(a.n2 + ubase + off);
+      o.e = ld_nt(a.E + ubase + off);
+      o.al = ld_nt(a.alpha + ubase + off);
+      o.be = ld_nt(a.beta + ubase + off);
     }
-    return v;
-  } else if (a.stage == 2) {
-    const cpx<T> s1 = ld_nt(a.stg + ubase + off);
-    double sr, si;
-    sol_at(sr, si);
-    const cpx<T> N1 = ld_nt(a.n1 + ubase + off);
-    const cpx<T> N2 = ld_nt(a.n2 + ubase + off);
-    const double e2 = ld_nt(a.E2 + ubase + off), z = ld_nt(a.zeta + ubase + off);
-    const double e = ld_nt(a.E + ubase + off), al = ld_nt(a.alpha + ubase + off);
-    const double be2 = 2.0 * ld_nt(a.beta + ubase + off);
-    const cpx<T> v = {(T)(e2 * (double)s1.re + z * (2.0 * nre - (double)N1.re)),
-                      (T)(e2 * (double)s1.im + z * (2.0 * nim - (double)N1.im))};
-    st_nt(a.stg2 + ubase + off, v);
-    const double ar = e * sr + al * (double)N1.re + be2 * ((double)N2.re + nre);
-    const double ai = e * si + al * (double)N1.im + be2 * ((double)N2.im + nim);
-    const cpx<T> hi = {(T)ar, (T)ai};
-    st_nt(a.acc + ubase + off, hi);
-    if (a.sol_lo) st_nt(a.n2 + ubase + off, {(T)(ar - (double)hi.re), (T)(ai - (double)hi.im)});  // low part
-    return v;
   } else {
     const cpx<T> ac = ld_nt(a.acc + ubase + off);
-    double ar = (double)ac.re, ai = (double)ac.im;
-    if (a.sol_lo) {
-      const cpx<T> al = ld_nt(a.n2 + ubase + off);
-      ar += (double)al.re;
-      ai += (double)al.im;
+    o.sr = (double)ac.re;
+    o.si = (double)ac.im;
+    if (kMixed && a.sol_lo) {
+      const cpx<T> al = ld_nt(a.n2 + ubase + off);   // the low part of acc
+      o.sr += (double)al.re;
+      o.si += (double)al.im;
     }
-    const double g = ld_nt(a.gam + ubase + off);
-    double sr = ar + g * nre, si = ai + g * nim;
+    o.g = ld_nt(a.gam + ubase + off);
+  }
+  return o;
+}
+
+template <typename T, int ST = -1>
+__device__ __forceinline__ cpx<T> etd_finish(const StageArgs<T>& a, long long ubase, uint32_t off,
+                                             int ikx, int iky, int ikz, cpx<T> N,
+                                             const EtdOps<T>& o, bool store = true) {
+  const int stage = ST >= 0 ? ST : a.stage;
+  constexpr bool kMixed = sizeof(T) == 4;
+  const double nre = (double)N.re, nim = (double)N.im;
+  if (stage == 0 || stage == 1) {
+    const cpx<T> v = {(T)(o.e2 * o.sr + o.z * nre), (T)(o.e2 * o.si + o.z * nim)};
+    if (store) {
+      if (stage == 0) {
+        st_nt(a.stg + ubase + off, v);
+        st_nt(a.n1 + ubase + off, N);
+      } else {
+        st_nt(a.stg2 + ubase + off, v);
+        st_nt(a.n2 + ubase + off, N);
+      }
+    }
+    return v;
+  } else if (stage == 2) {
+    const double be2 = 2.0 * o.be;
+    const cpx<T> v = {(T)(o.e2 * (double)o.s1.re + o.z * (2.0 * nre - (double)o.n1.re)),
+                      (T)(o.e2 * (double)o.s1.im + o.z * (2.0 * nim - (double)o.n1.im))};
+    const double ar = o.e * o.sr + o.al * (double)o.n1.re + be2 * ((double)o.n2.re + nre);
+    const double ai = o.e * o.si + o.al * (double)o.n1.im + be2 * ((double)o.n2.im + nim);
+    const cpx<T> hi = {(T)ar, (T)ai};
+    if (store) {
+      st_nt(a.stg2 + ubase + off, v);
+      st_nt(a.acc + ubase + off, hi);
+      if (kMixed && a.sol_lo)   // the low part of acc
+        st_nt(a.n2 + ubase + off, {(T)(ar - (double)hi.re), (T)(ai - (double)hi.im)});
+    }
+    return v;
+  } else {
+    double sr = o.sr + o.g * nre, si = o.si + o.g * nim;
     if (a.filtered) {
       const double f = filter_value(a.g, a.f_order, a.f_innerK, a.f_decay, ikx, iky, ikz);
       sr *= f;
       si *= f;
     }
     const cpx<T> hi = {(T)sr, (T)si};
-    st_nt(a.sol + ubase + off, hi);
-    if (a.sol_lo) st_nt(a.sol_lo + ubase + off, {(T)(sr - (double)hi.re), (T)(si - (double)hi.im)});
+    if (store) {
+      st_nt(a.sol + ubase + off, hi);
+      if (kMixed && a.sol_lo)
+        st_nt(a.sol_lo + ubase + off, {(T)(sr - (double)hi.re), (T)(si - (double)hi.im)});
+    }
     return hi;
   }
+}
+
+template <typename T>
+__device__ __forceinline__ cpx<T> etdrk4_update(const StageArgs<T>& a, long long ubase,
+                                                uint32_t off, int ikx, int iky, int ikz, cpx<T> N) {
+  const EtdOps<T> o = etd_load(a, ubase, off);
+  return etd_finish(a, ubase, off, ikx, iky, ikz, N, o);
 }
 
 // Stage update at spectral element ubase + off with the stage's nonlinear term N;

@@ -239,11 +239,11 @@ void col_kernel(ColArgs<T> a) {
       constexpr bool kPre = sizeof(T) == 4 || R <= 4;
       Rk4Ops<T> ops[kPre ? R : 1];
       const bool rk4 = a.st.kind == EPI_RK4;
-      // The RK4 stage is uniform: dispatch on it once, so that the operand loads of a
-      // chunk of elements form one straight-line block and go out together (one memory
-      // round trip per chunk, not one per element).  FP64 radix >= 8 loads in two chunks
-      // (registers).
-      auto rk4_dispatch = [&](auto body) {
+      // The stage is uniform: dispatch on it once, so that the operand loads of a chunk
+      // of elements form one straight-line block and go out together (one memory round
+      // trip per chunk, not one per element).  FP64 radix >= 8 (RK4) and radix >= 8
+      // (ETDRK4, up to 9 operands per element) load in chunks of R/2 and 4 elements.
+      auto stage_dispatch = [&](auto body) {
         switch (a.st.stage) {
           case 0: body(std::integral_constant<int, 0>{}); break;
           case 1: body(std::integral_constant<int, 1>{}); break;
@@ -253,7 +253,7 @@ void col_kernel(ColArgs<T> a) {
       };
       if constexpr (kPre) {
         if (rk4) {
-          rk4_dispatch([&](auto stc) {
+          stage_dispatch([&](auto stc) {
             constexpr int ST = decltype(stc)::value;
 #pragma unroll
             for (int q = 0; q < R; ++q) ops[q] = rk4_load<T, ST>(a.st, ubase, off(q));
@@ -264,7 +264,7 @@ void col_kernel(ColArgs<T> a) {
       // invalid (tail) lanes compute on a clamped column and store nothing
       if (rk4) {
         const OpParams opm = a.st.op[member];
-        rk4_dispatch([&](auto stc) {
+        stage_dispatch([&](auto stc) {
           constexpr int ST = decltype(stc)::value;
           constexpr int kChunk = (!kPre && sizeof(T) == 8 && R >= 8) ? R / 2 : R;
 #pragma unroll
@@ -287,20 +287,30 @@ void col_kernel(ColArgs<T> a) {
             }
           }
         });
-      } else {
+      } else if (a.st.kind == EPI_ETDRK4) {
+        stage_dispatch([&](auto stc) {
+          constexpr int ST = decltype(stc)::value;
+          constexpr int kChunk = R < 4 ? R : 4;
 #pragma unroll
-        for (int q = 0; q < R; ++q) {
-          const int ia = j + q * P;
-          const int iky = a.axis == 2 ? by : ia;
-          const int ikz = a.axis == 2 ? ia : iklast_col;
-          if constexpr (MODE == COL_STAGE) {
-            if (!valid) continue;
-            if (a.st.kind == EPI_STORE) {
-              (a.out0 + ubase)[off(q)] = v[q];
-              continue;
+          for (int q0 = 0; q0 < R; q0 += kChunk) {
+            EtdOps<T> o[kChunk];
+#pragma unroll
+            for (int i = 0; i < kChunk; ++i) o[i] = etd_load<T, ST>(a.st, ubase, off(q0 + i));
+#pragma unroll
+            for (int i = 0; i < kChunk; ++i) {
+              const int q = q0 + i;
+              const int ia = j + q * P;
+              const int iky = a.axis == 2 ? by : ia;
+              const int ikz = a.axis == 2 ? ia : iklast_col;
+              v[q] = etd_finish<T, ST>(a.st, ubase, off(q), ikx_col, iky, ikz, v[q], o[i], valid);
             }
           }
-          if (valid) v[q] = stage_epilogue(a.st, ubase, off(q), member, ikx_col, iky, ikz, v[q]);
+        });
+      } else if constexpr (MODE == COL_STAGE) {
+        // plain store of the transformed line (calcN! without a stage update)
+        if (valid) {
+#pragma unroll
+          for (int q = 0; q < R; ++q) (a.out0 + ubase)[off(q)] = v[q];
         }
       }
       if constexpr (MODE == COL_STAGE_D1) {
