@@ -592,18 +592,23 @@ void row_kernel(RowArgs<T> a) {
 #pragma unroll
     for (int q = 0; q < R; ++q) lds[lay.idx(c, j + q * P)] = f[q];
     __syncthreads();
+    // the post-processed outputs of all elements before any store: a twiddle load placed
+    // after a store could not be issued ahead of it (the arrays may alias), which would
+    // cost one memory round trip per element
+    cpx<T> Xs[R];
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+      const int k = j + q * P;
+      const cpx<T> w = a.tw[k * a.twstride];
+      const cpx<T> Zm = lds[lay.idx(c, M - k)];   // row entry M exists (k = 0 discards it)
+      Xs[q] = k == 0 ? cpx<T>{f[q].re + f[q].im, (T)0} : r2c_post(f[q], Zm, w);
+    }
     if (valid) {
       const int iky = (int)(row % a.ny), ikz = (int)(row / a.ny);
 #pragma unroll
       for (int q = 0; q < R; ++q) {
         const int k = j + q * P;
-        cpx<T> X;
-        if (k == 0) {
-          X = {f[q].re + f[q].im, (T)0};
-        } else {
-          const cpx<T> Zm = lds[lay.idx(c, M - k)];
-          X = r2c_post(f[q], Zm, a.tw[k * a.twstride]);
-        }
+        const cpx<T> X = Xs[q];
         // only 1D problems finish a stage in the x kernel (x is their last forward axis)
         constexpr bool kEpi = MODE == ROW_ADV && NIN == 1;
         bool epi = false;
@@ -733,20 +738,19 @@ void row_adv_par_kernel(RowArgs<T> a) {
 #pragma unroll
   for (int q = 0; q < R; ++q) lds[lay.idx(c, j + q * P)] = f[q];
   __syncthreads();
-  if (valid) {
+  // all outputs before any store (twiddle loads ahead of the stores, as in row_kernel)
+  cpx<T> Xs[R];
 #pragma unroll
-    for (int q = 0; q < R; ++q) {
-      const int k = j + q * P;
-      cpx<T> X;
-      if (k == 0) {
-        X = {f[q].re + f[q].im, (T)0};
-        (a.out + sub)[so_st + ((uint32_t)M << lb)] = cpx<T>{f[q].re - f[q].im, (T)0};
-      } else {
-        const cpx<T> Zm = lds[lay.idx(c, M - k)];
-        X = r2c_post(f[q], Zm, a.tw[k * a.twstride]);
-      }
-      (a.out + sub)[so_st + ((uint32_t)k << lb)] = X;
-    }
+  for (int q = 0; q < R; ++q) {
+    const int k = j + q * P;
+    const cpx<T> w = a.tw[k * a.twstride];
+    const cpx<T> Zm = lds[lay.idx(c, M - k)];   // row entry M exists (k = 0 discards it)
+    Xs[q] = k == 0 ? cpx<T>{f[q].re + f[q].im, (T)0} : r2c_post(f[q], Zm, w);
+  }
+  if (valid) {
+    if (j == 0) (a.out + sub)[so_st + ((uint32_t)M << lb)] = cpx<T>{f[0].re - f[0].im, (T)0};
+#pragma unroll
+    for (int q = 0; q < R; ++q) (a.out + sub)[so_st + ((uint32_t)(j + q * P) << lb)] = Xs[q];
   }
 }
 
