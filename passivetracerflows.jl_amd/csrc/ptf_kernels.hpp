@@ -90,8 +90,13 @@ __device__ __forceinline__ void xcd_remap(int& bx, int& by, int& bz) {
   bz = (int)(logical / (gx * gy));
 }
 
+// Lines of up to 512 points keep their twiddle table in LDS: the transform's twiddle
+// reads then come back with its LDS reads instead of from L1/L2 after every exchange.
+template <int N, int R>
+constexpr bool col_tw_lds() { return N > R && N <= 512; }
+
 template <typename T, int N, int R, int TX>
-constexpr int col_lds_elems() { return (N > R) ? N * TX : 1; }
+constexpr int col_lds_elems() { return ((N > R) ? N * TX : 1) + (col_tw_lds<N, R>() ? N : 0); }
 
 template <typename T, int N, int R, int TX, int MODE, int DIR, int OCC = 0>
 __global__ __attribute__((amdgpu_flat_work_group_size(1, TX * (N / R)),
@@ -102,8 +107,14 @@ void col_kernel(ColArgs<T> a) {
   constexpr int P = N / R;
   using Lay = LayInterleaved<TX>;
   const Lay lay{};
-  __shared__ cpx<T> lds[col_lds_elems<T, N, R, TX>()];
+  __shared__ cpx<T> lds[(N > R) ? N * TX : 1];
+  __shared__ cpx<T> twl[col_tw_lds<N, R>() ? N : 1];
   const int t = threadIdx.x;
+  if constexpr (col_tw_lds<N, R>()) {
+    // twl[m] = exp(-2 pi i m / N); the first barrier of every transform orders these
+    // writes before its first twiddle read
+    for (int i = t; i < N; i += TX * P) twl[i] = a.tw[i * a.twstride];
+  }
   // lane t = rb + 2^lb (c + TX jh): the brick's planes vary fastest, then the
   // columns, so a wavefront reads 2^lb * TX contiguous values; the line
   // element of register q is i = j + q P with j = rb + 2^lb jh (P % 2^lb == 0)
@@ -113,6 +124,12 @@ void col_kernel(ColArgs<T> a) {
   const int c = rest % TX;
   const int jh = rest / TX;
   const int j = rb + (jh << lb);
+  // the line transform in direction D with this block's twiddle source
+  auto fft = [&](auto dir_c, cpx<T>(&x)[R]) {
+    constexpr int D = decltype(dir_c)::value;
+    if constexpr (col_tw_lds<N, R>()) line_fft<T, N, R, D, Lay>(x, lds, lay, c, j, twl, 1);
+    else line_fft<T, N, R, D, Lay>(x, lds, lay, c, j, a.tw, a.twstride);
+  };
   int bx, by, member;
   xcd_remap(bx, by, member);
   const int kx = bx * TX + c;
@@ -151,7 +168,7 @@ void col_kernel(ColArgs<T> a) {
           v[q] = {(T)(l * p.im), (T)(-(l * p.re))};             // -i l p
         }
       }
-      line_fft<T, N, R, +1, Lay>(v, lds, lay, c, j, a.tw, a.twstride);
+      fft(std::integral_constant<int, 1>{}, v);
       cpx<T>* o = (pass == 0 ? a.out1 : a.out0) + ubase;
       if (valid) {
 #pragma unroll
@@ -171,7 +188,7 @@ void col_kernel(ColArgs<T> a) {
     }
 
     if constexpr (MODE == COL_PLAIN) {
-      line_fft<T, N, R, DIR, Lay>(v, lds, lay, c, j, a.tw, a.twstride);
+      fft(std::integral_constant<int, DIR>{}, v);
       cpx<T>* o = a.out0 + ubase;
       if (valid) {
 #pragma unroll
@@ -198,7 +215,7 @@ void col_kernel(ColArgs<T> a) {
         const T k = (T)(fftfreq_index(j + q * P, N) * a.kscale);
         v[q] = times_ik(v[q], k);
       }
-      line_fft<T, N, R, +1, Lay>(v, lds, lay, c, j, a.tw, a.twstride);
+      fft(std::integral_constant<int, 1>{}, v);
       {
         cpx<T>* o = a.out1 + ubase;
         if (valid) {
@@ -211,7 +228,7 @@ void col_kernel(ColArgs<T> a) {
         if constexpr (kKeep) v[q] = u[q];
         else v[q] = in0[off(q)];
       }
-      line_fft<T, N, R, +1, Lay>(v, lds, lay, c, j, a.tw, a.twstride);
+      fft(std::integral_constant<int, 1>{}, v);
       {
         cpx<T>* o = a.out0 + ubase;
         if (valid) {
@@ -226,7 +243,7 @@ void col_kernel(ColArgs<T> a) {
           if constexpr (kKeep) v[q] = p2[q];
           else v[q] = ld_nt(in1 + off(q));
         }
-        line_fft<T, N, R, +1, Lay>(v, lds, lay, c, j, a.tw, a.twstride);
+        fft(std::integral_constant<int, 1>{}, v);
         cpx<T>* o = a.out2 + ubase;
         if (valid) {
 #pragma unroll
@@ -260,7 +277,7 @@ void col_kernel(ColArgs<T> a) {
           });
         }
       }
-      line_fft<T, N, R, -1, Lay>(v, lds, lay, c, j, a.tw, a.twstride);
+      fft(std::integral_constant<int, -1>{}, v);
       // invalid (tail) lanes compute on a clamped column and store nothing
       if (rk4) {
         const OpParams opm = a.st.op[member];
@@ -331,7 +348,7 @@ void col_kernel(ColArgs<T> a) {
           const T k = (T)(fftfreq_index(j + q * P, N) * a.kscale);
           v[q] = times_ik(v[q], k);
         }
-        line_fft<T, N, R, +1, Lay>(v, lds, lay, c, j, a.tw, a.twstride);
+        fft(std::integral_constant<int, 1>{}, v);
         {
           cpx<T>* op = a.out1 + ubase;
           if (valid) {
@@ -344,7 +361,7 @@ void col_kernel(ColArgs<T> a) {
           if constexpr (kKeep) v[q] = u[q];
           else v[q] = w[q];
         }
-        line_fft<T, N, R, +1, Lay>(v, lds, lay, c, j, a.tw, a.twstride);
+        fft(std::integral_constant<int, 1>{}, v);
         cpx<T>* op = a.out0 + ubase;
         if (valid) {
 #pragma unroll

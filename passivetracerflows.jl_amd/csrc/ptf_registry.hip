@@ -44,7 +44,7 @@ template <typename T, int N, int R, int TX, int KIND, int OCC>
 const void* colp() {
   constexpr int P = N / R;
   constexpr bool ok = R <= N && (R >= 4) && TX * P >= 16 && TX * P <= 1024 &&
-                      (long long)N * TX * (long long)sizeof(cpx<T>) <= 163840;
+                      (long long)col_lds_elems<T, N, R, TX>() * (long long)sizeof(cpx<T>) <= 163840;
   if constexpr (ok) {
     return reinterpret_cast<const void*>(&col_kernel<T, N, R, TX, col_mode(KIND), col_dir(KIND), OCC>);
   } else {
