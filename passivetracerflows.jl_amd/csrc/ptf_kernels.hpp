@@ -479,11 +479,19 @@ __device__ __forceinline__ long long brick_row_base(long long row0, int lb, int 
   return (((row0 >> lb) * len) << lb) + (row0 & ((1LL << lb) - 1));
 }
 
+// Rows of up to 512 points keep the twiddle table exp(-2 pi i k / nx), k < nx, in LDS
+// (the half-length transform reads every other entry, the real pre/post-processing
+// the first nx/2).
+template <int NX>
+constexpr bool row_tw_lds() { return NX <= 512; }
+template <typename T, int NX>
+constexpr int row_tw_bytes() { return row_tw_lds<NX>() ? NX * (int)sizeof(cpx<T>) : 0; }
+
 template <typename T, int NX, int R, int NL, int MODE, int NIN, int OCC = 0>
 __global__ __attribute__((amdgpu_flat_work_group_size(1, NL * ((NX / 2) / R)),
                           amdgpu_waves_per_eu(
-                              min_waves(NL * ((NX / 2) / R), (int)(NL * (NX / 2 + 1) * sizeof(cpx<T>)), OCC),
-                              lds_waves(NL * ((NX / 2) / R), (int)(NL * (NX / 2 + 1) * sizeof(cpx<T>))))))
+                              min_waves(NL * ((NX / 2) / R), (int)(NL * (NX / 2 + 1) * sizeof(cpx<T>)) + row_tw_bytes<T, NX>(), OCC),
+                              lds_waves(NL * ((NX / 2) / R), (int)(NL * (NX / 2 + 1) * sizeof(cpx<T>)) + row_tw_bytes<T, NX>()))))
 void row_kernel(RowArgs<T> a) {
   constexpr int M = NX / 2;
   constexpr int NXH = M + 1;
@@ -495,7 +503,12 @@ void row_kernel(RowArgs<T> a) {
   using Lay = LayStrided;
   const Lay lay = a.lb ? Lay{1, NL} : Lay{M + PAD, 1};
   __shared__ cpx<T> lds[NL * (M + PAD)];
+  __shared__ cpx<T> twl[row_tw_lds<NX>() ? NX : 1];
   const int t = threadIdx.x;
+  if constexpr (row_tw_lds<NX>()) {
+    // read only after the barriers of the first row exchange or transform
+    for (int i = t; i < NX; i += (int)blockDim.x) twl[i] = a.tw[i * a.twstride];
+  }
   // lane t = rb + 2^lt (j + P g), 2^lt = min(2^lb, NL): the block's rows of a brick vary
   // fastest.  With part of a brick per block (NL < 2^lb) the sibling blocks of a brick,
   // which share its cache lines, are renumbered onto one XCD.
@@ -532,9 +545,10 @@ void row_kernel(RowArgs<T> a) {
 #pragma unroll
     for (int q = 0; q < R; ++q) {
       const int k = j + q * P;
-      z[q] = c2r_pre(z[q], zm[q], a.tw[k * a.twstride], s, k == 0);
+      z[q] = c2r_pre(z[q], zm[q], (row_tw_lds<NX>() ? twl[k] : a.tw[k * a.twstride]), s, k == 0);
     }
-    line_fft<T, M, R, +1, Lay>(z, lds, lay, c, j, a.tw, 2 * a.twstride);
+    if constexpr (row_tw_lds<NX>()) line_fft<T, M, R, +1, Lay>(z, lds, lay, c, j, twl, 2);
+    else line_fft<T, M, R, +1, Lay>(z, lds, lay, c, j, a.tw, 2 * a.twstride);
     if (valid) {
       cpx<T>* out = reinterpret_cast<cpx<T>*>(a.rout + pub);
 #pragma unroll
@@ -586,9 +600,10 @@ void row_kernel(RowArgs<T> a) {
             Xk = times_ik(Xk, (T)(k * a.kscale_x));
             Xm = times_ik(Xm, (T)((M - k) * a.kscale_x));
           }
-          z[q] = c2r_pre(Xk, Xm, a.tw[k * a.twstride], s, k == 0);
+          z[q] = c2r_pre(Xk, Xm, (row_tw_lds<NX>() ? twl[k] : a.tw[k * a.twstride]), s, k == 0);
         }
-        line_fft<T, M, R, +1, Lay>(z, lds, lay, c, j, a.tw, 2 * a.twstride);
+        if constexpr (row_tw_lds<NX>()) line_fft<T, M, R, +1, Lay>(z, lds, lay, c, j, twl, 2);
+        else line_fft<T, M, R, +1, Lay>(z, lds, lay, c, j, a.tw, 2 * a.twstride);
 #pragma unroll
         for (int q = 0; q < R; ++q) {
           cpx<T> uu;
@@ -603,7 +618,8 @@ void row_kernel(RowArgs<T> a) {
       }
     }
     // forward half-length FFT of f[2m] + i f[2m+1]
-    line_fft<T, M, R, -1, Lay>(f, lds, lay, c, j, a.tw, 2 * a.twstride);
+    if constexpr (row_tw_lds<NX>()) line_fft<T, M, R, -1, Lay>(f, lds, lay, c, j, twl, 2);
+    else line_fft<T, M, R, -1, Lay>(f, lds, lay, c, j, a.tw, 2 * a.twstride);
     // exchange Z[M-k] through LDS
     __syncthreads();
 #pragma unroll
@@ -616,7 +632,7 @@ void row_kernel(RowArgs<T> a) {
 #pragma unroll
     for (int q = 0; q < R; ++q) {
       const int k = j + q * P;
-      const cpx<T> w = a.tw[k * a.twstride];
+      const cpx<T> w = (row_tw_lds<NX>() ? twl[k] : a.tw[k * a.twstride]);
       const cpx<T> Zm = lds[lay.idx(c, M - k)];   // row entry M exists (k = 0 discards it)
       Xs[q] = k == 0 ? cpx<T>{f[q].re + f[q].im, (T)0} : r2c_post(f[q], Zm, w);
     }
@@ -660,7 +676,7 @@ void row_kernel(RowArgs<T> a) {
 template <typename T, int NX, int R, int NL, int NIN>
 constexpr int adv_par_threads() { return NIN * NL * ((NX / 2) / R); }
 template <typename T, int NX, int R, int NL, int NIN>
-constexpr int adv_par_lds() { return NIN * NL * (NX / 2 + 1) * (int)sizeof(cpx<T>); }
+constexpr int adv_par_lds() { return NIN * NL * (NX / 2 + 1) * (int)sizeof(cpx<T>) + row_tw_bytes<T, NX>(); }
 
 template <typename T, int NX, int R, int NL, int NIN, int OCC = 0>
 __global__ __attribute__((amdgpu_flat_work_group_size(1, adv_par_threads<T, NX, R, NL, NIN>()),
@@ -677,6 +693,11 @@ void row_adv_par_kernel(RowArgs<T> a) {
   using Lay = LayStrided;
   const Lay lay = a.lb ? Lay{1, NL} : Lay{M + 1, 1};
   __shared__ cpx<T> lds_all[NIN * NL * (M + 1)];
+  __shared__ cpx<T> twl[row_tw_lds<NX>() ? NX : 1];
+  if constexpr (row_tw_lds<NX>()) {
+    // every group fills (before any group ends); read after the first row exchange
+    for (int i = (int)threadIdx.x; i < NX; i += (int)blockDim.x) twl[i] = a.tw[i * a.twstride];
+  }
   const int grp = (int)threadIdx.x / G;   // uniform over a wavefront
   const int t = (int)threadIdx.x - grp * G;
   cpx<T>* lds = lds_all + grp * (NL * (M + 1));
@@ -724,9 +745,10 @@ void row_adv_par_kernel(RowArgs<T> a) {
       Xk = times_ik(Xk, (T)(k * a.kscale_x));
       Xm = times_ik(Xm, (T)((M - k) * a.kscale_x));
     }
-    z[q] = c2r_pre(Xk, Xm, a.tw[k * a.twstride], s, k == 0);
+    z[q] = c2r_pre(Xk, Xm, (row_tw_lds<NX>() ? twl[k] : a.tw[k * a.twstride]), s, k == 0);
   }
-  line_fft<T, M, R, +1, Lay>(z, lds, lay, c, j, a.tw, 2 * a.twstride);
+  if constexpr (row_tw_lds<NX>()) line_fft<T, M, R, +1, Lay>(z, lds, lay, c, j, twl, 2);
+  else line_fft<T, M, R, +1, Lay>(z, lds, lay, c, j, a.tw, 2 * a.twstride);
   cpx<T> f[R];
 #pragma unroll
   for (int q = 0; q < R; ++q) f[q] = {fv[q].re * z[q].re, fv[q].im * z[q].im};
@@ -750,7 +772,8 @@ void row_adv_par_kernel(RowArgs<T> a) {
     f[q] = v;
   }
   // forward half-length FFT of f[2m] + i f[2m+1] (barriers now among group 0 only)
-  line_fft<T, M, R, -1, Lay>(f, lds, lay, c, j, a.tw, 2 * a.twstride);
+  if constexpr (row_tw_lds<NX>()) line_fft<T, M, R, -1, Lay>(f, lds, lay, c, j, twl, 2);
+  else line_fft<T, M, R, -1, Lay>(f, lds, lay, c, j, a.tw, 2 * a.twstride);
   __syncthreads();
 #pragma unroll
   for (int q = 0; q < R; ++q) lds[lay.idx(c, j + q * P)] = f[q];
@@ -760,7 +783,7 @@ void row_adv_par_kernel(RowArgs<T> a) {
 #pragma unroll
   for (int q = 0; q < R; ++q) {
     const int k = j + q * P;
-    const cpx<T> w = a.tw[k * a.twstride];
+    const cpx<T> w = (row_tw_lds<NX>() ? twl[k] : a.tw[k * a.twstride]);
     const cpx<T> Zm = lds[lay.idx(c, M - k)];   // row entry M exists (k = 0 discards it)
     Xs[q] = k == 0 ? cpx<T>{f[q].re + f[q].im, (T)0} : r2c_post(f[q], Zm, w);
   }

@@ -57,7 +57,7 @@ const void* rowp() {
   constexpr int M = NX / 2;
   constexpr int P = M / R;
   constexpr bool ok = R <= M && (R >= 4) && NL * P >= 16 && NL * P <= 1024 &&
-                      (long long)NL * (M + 1) * (long long)sizeof(cpx<T>) <= 163840;
+                      (long long)NL * (M + 1) * (long long)sizeof(cpx<T>) + row_tw_bytes<T, NX>() <= 163840;
   if constexpr (ok) {
     return reinterpret_cast<const void*>(&row_kernel<T, NX, R, NL, row_mode(KIND), row_nin(KIND), OCC>);
   } else {
@@ -73,7 +73,7 @@ const void* rowpar() {
     constexpr int M = NX / 2;
     constexpr int P = M / R;
     constexpr bool ok = R <= M && R >= 4 && (NL * P) % 64 == 0 && NIN * NL * P <= 1024 &&
-                        (long long)NIN * NL * (M + 1) * (long long)sizeof(cpx<T>) <= 163840;
+                        adv_par_lds<T, NX, R, NL, NIN>() <= 163840;
     if constexpr (ok) {
       return reinterpret_cast<const void*>(&row_adv_par_kernel<T, NX, R, NL, NIN, OCC>);
     } else {
