@@ -1102,15 +1102,28 @@ static int time_slot(Group& g, Slot& s, int reps, hipStream_t st, double* avg_ms
   return PTF_OK;
 }
 
-// autotune: for every distinct (kind, N, lines) pick the fastest valid (R, W)
+// autotune: for every distinct (kind, N, lines) pick the fastest valid (R, W, form).  A
+// candidate is scored by the sum of its times over the distinct launches of that key
+// (the four stages of a stage kernel read different operands), each timed in isolation.
+static std::string tune_key(const Slot& s) {
+  return std::to_string(s.kind) + ":" + std::to_string(s.N) + ":" + std::to_string(s.lines) +
+         ":" + std::to_string(s.members);
+}
+
 static int autotune_group(Group& g, hipStream_t st) {
   std::vector<Slot*> all = step_slots(g);
   std::vector<std::string> done;
   for (Slot* s : all) {
-    const std::string key = std::to_string(s->kind) + ":" + std::to_string(s->N) + ":" +
-                            std::to_string(s->lines) + ":" + std::to_string(s->members);
+    const std::string key = tune_key(*s);
     if (std::find(done.begin(), done.end(), key) != done.end()) continue;
     done.push_back(key);
+    std::vector<Slot*> same;   // launches of this key with distinct arguments
+    for (Slot* o : all) {
+      if (tune_key(*o) != key) continue;
+      bool dup = false;
+      for (Slot* u : same) dup = dup || u->args == o->args;
+      if (!dup) same.push_back(o);
+    }
     const int* widths = s->is_col() ? kColWidths : kRowWidths;
     const int nw = s->is_col() ? kNumColWidths : kNumRowWidths;
     int bestR = s->R, bestW = s->W, bestOcc = s->occ;
@@ -1121,15 +1134,24 @@ static int autotune_group(Group& g, hipStream_t st) {
       for (int occ : {0, 3}) {
         for (int R : kRadices) {
           for (int wi = 0; wi < nw; ++wi) {
-            Slot trial = *s;
-            if (!choose_variant(trial, R, widths[wi], occ, par)) continue;
-            double ms = 0;
-            CHK(time_slot(g, trial, 5, st, &ms));
+            double tot = 0;
+            bool ok = true;
+            for (Slot* o : same) {
+              Slot trial = *o;
+              if (!choose_variant(trial, R, widths[wi], occ, par)) {
+                ok = false;
+                break;
+              }
+              double ms = 0;
+              CHK(time_slot(g, trial, 5, st, &ms));
+              tot += ms;
+            }
+            if (!ok) continue;
             if (verbose)
-              std::fprintf(stderr, "autotune %s N=%d R=%d W=%d occ=%d par=%d: %.4f ms\n",
-                           kind_name(trial.kind), trial.N, R, widths[wi], occ, par, ms);
-            if (ms < best) {
-              best = ms;
+              std::fprintf(stderr, "autotune %s N=%d R=%d W=%d occ=%d par=%d: %.4f ms (%d launches)\n",
+                           kind_name(s->kind), s->N, R, widths[wi], occ, par, tot, (int)same.size());
+            if (tot < best) {
+              best = tot;
               bestR = R;
               bestW = widths[wi];
               bestOcc = occ;
@@ -1139,11 +1161,8 @@ static int autotune_group(Group& g, hipStream_t st) {
         }
       }
     }
-    for (Slot* o : all) {
-      const std::string k2 = std::to_string(o->kind) + ":" + std::to_string(o->N) + ":" +
-                             std::to_string(o->lines) + ":" + std::to_string(o->members);
-      if (k2 == key) choose_variant(*o, bestR, bestW, bestOcc, bestPar);
-    }
+    for (Slot* o : all)
+      if (tune_key(*o) == key) choose_variant(*o, bestR, bestW, bestOcc, bestPar);
   }
   return PTF_OK;
 }
