@@ -90,10 +90,10 @@ __device__ __forceinline__ void xcd_remap(int& bx, int& by, int& bz) {
   bz = (int)(logical / (gx * gy));
 }
 
-// Lines of up to 512 points keep their twiddle table in LDS: the transform's twiddle
+// Lines of up to 2048 points keep their twiddle table in LDS: the transform's twiddle
 // reads then come back with its LDS reads instead of from L1/L2 after every exchange.
 template <int N, int R>
-constexpr bool col_tw_lds() { return N > R && N <= 512; }
+constexpr bool col_tw_lds() { return N > R && N <= 2048; }
 
 template <typename T, int N, int R, int TX>
 constexpr int col_lds_elems() { return ((N > R) ? N * TX : 1) + (col_tw_lds<N, R>() ? N : 0); }
@@ -479,11 +479,11 @@ __device__ __forceinline__ long long brick_row_base(long long row0, int lb, int 
   return (((row0 >> lb) * len) << lb) + (row0 & ((1LL << lb) - 1));
 }
 
-// Rows of up to 512 points keep the twiddle table exp(-2 pi i k / nx), k < nx, in LDS
+// Rows of up to 2048 points keep the twiddle table exp(-2 pi i k / nx), k < nx, in LDS
 // (the half-length transform reads every other entry, the real pre/post-processing
 // the first nx/2).
 template <int NX>
-constexpr bool row_tw_lds() { return NX <= 512; }
+constexpr bool row_tw_lds() { return NX <= 2048; }
 template <typename T, int NX>
 constexpr int row_tw_bytes() { return row_tw_lds<NX>() ? NX * (int)sizeof(cpx<T>) : 0; }
 
