@@ -258,8 +258,8 @@ void col_kernel(ColArgs<T> a) {
       const bool rk4 = a.st.kind == EPI_RK4;
       // The stage is uniform: dispatch on it once, so that the operand loads of a chunk
       // of elements form one straight-line block and go out together (one memory round
-      // trip per chunk, not one per element).  FP64 radix >= 8 (RK4) and radix >= 8
-      // (ETDRK4, up to 9 operands per element) load in chunks of R/2 and 4 elements.
+      // trip per chunk, not one per element).  FP64 radix >= 8 (RK4) loads in two chunks;
+      // ETDRK4 (up to 9 operands per element) in chunks of 4 or 2 elements.
       auto stage_dispatch = [&](auto body) {
         switch (a.st.stage) {
           case 0: body(std::integral_constant<int, 0>{}); break;
@@ -307,7 +307,8 @@ void col_kernel(ColArgs<T> a) {
       } else if (a.st.kind == EPI_ETDRK4) {
         stage_dispatch([&](auto stc) {
           constexpr int ST = decltype(stc)::value;
-          constexpr int kChunk = R < 4 ? R : 4;
+          // up to 9 operands per element: 4 elements at a time, 2 under the 3-waves/SIMD budget
+          constexpr int kChunk = OCC > 0 ? 2 : (R < 4 ? R : 4);
 #pragma unroll
           for (int q0 = 0; q0 < R; q0 += kChunk) {
             EtdOps<T> o[kChunk];
