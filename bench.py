@@ -32,6 +32,9 @@ for p in (ROOT, os.path.join(ROOT, "passivetracerflows.jl_amd")):
         sys.path.insert(0, p)
 
 METRIC = "grid-point-steps/s, FP64 plain vs FP32 batched at relL2≤1e-5; %HBM roofline"
+# the autotuner's per-slot choices, stored by the first tuned run and applied by later ones
+# (so a profiled run times the same variants as the benchmark); --retune refreshes them
+TUNING = os.path.join(ROOT, "passivetracerflows.jl_amd", "passivetracerflows", "tuning_gfx950.json")
 HBM_PEAK_GBS = 8000.0
 # fixed yardstick for the whole step: every FFT axis pass reads and writes its fields once,
 # flows read once per stage, RK4 state 4/6/6/5 S-fields per stage: 77 S + 12 R per step
@@ -200,7 +203,25 @@ def time_steps(stepfn, steps):
     return time.perf_counter() - t0
 
 
-def headline(dev, steps, warmup, T=np.float64, autotune=True):
+def _tune(obj, autotune):
+    """autotune: False (default variants), "cache" (stored choice, else tune and store) or
+    "retune" (tune and overwrite the stored choice)."""
+    if not autotune:
+        return "default"
+    if autotune == "retune":
+        from passivetracerflows import tuning
+        data = tuning.load(TUNING)
+        data.pop(obj.tuning_key() if hasattr(obj, "tuning_key") else _problem_key(obj), None)
+        tuning.store(TUNING, data)
+    return obj.autotune(cache=TUNING)
+
+
+def _problem_key(p):
+    from passivetracerflows.fourierflows import tuning_key
+    return tuning_key(p)
+
+
+def headline(dev, steps, warmup, T=np.float64, autotune="cache"):
     import passivetracerflows as ptf
     from passivetracerflows import workloads as W
     TAD = ptf.TracerAdvectionDiffusion
@@ -210,8 +231,7 @@ def headline(dev, steps, warmup, T=np.float64, autotune=True):
                                                      w=lambda x, y, z: w),
                        nx=256, κ=cfg["kappa"], dt=cfg["dt"], stepper="RK4", T=T)
     TAD.set_c_(prob, cfg["c0"])
-    if autotune:
-        prob.autotune()
+    prob.tuned = _tune(prob, autotune)
     m0 = TAD.mass(prob)
     v0 = TAD.variance(prob)
     ptf.stepforward_(prob, warmup)
@@ -267,6 +287,17 @@ def dominant(rows):
     return name, f, avg_ms, avg_bytes, fams
 
 
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(cfg, steps):
     from oracle import tad as O
     cores = min(16, os.cpu_count() or 1)
@@ -279,7 +310,7 @@ def cpu_baseline(cfg, steps):
     p.stepforward(steps)
     dt = time.perf_counter() - t0
     return dict(value=gps(g.npoints, steps, dt), unit="grid-point-steps/s", cores=cores,
-                kind="port",
+                kind="port", cpu=cpu_model(), nproc=os.cpu_count(),
                 sample=f"FP64 oracle restatement (scipy.fft pocketfft, workers={cores}; numpy "
                        f"elementwise single-threaded), 3D ABC 256^3 RK4, {steps} steps after 1 "
                        f"warm-up step, {dt:.1f} s")
@@ -314,7 +345,41 @@ def mixed_errors(TAD, p32, p64, c0):
                 variance_err_rel=abs(TAD.variance(p32) - v64) / v64)
 
 
-def extra_forms(dev, steps, ens_steps):
+def full_length_mixed_errors(dev, autotune="cache"):
+    """The mixed form against the FP64 form after each workload's configured step count
+    (3D 256^3 x 500, 2D 256^2 x 2000, 2D 2048^2 ETDRK4 x 1000): relative L2 of sol, mass
+    error relative to sum|c0|, variance error relative to the variance."""
+    import passivetracerflows as ptf
+    from passivetracerflows import workloads as W
+    TAD = ptf.TracerAdvectionDiffusion
+    out = {}
+    cases = (
+        ("abc3d_256_rk4", W.abc_3d(256),
+         lambda cfg, T: TAD.Problem(dev, ptf.ThreeDAdvectingFlow(*[(lambda *a, f=f: f) for f in cfg["flow"]]),
+                                    nx=256, κ=cfg["kappa"], dt=cfg["dt"], T=T)),
+        ("cellular2d_256_rk4", W.cellular_2d(256),
+         lambda cfg, T: TAD.Problem(dev, ptf.TwoDAdvectingFlow(*[(lambda *a, f=f: f) for f in cfg["flow"]]),
+                                    nx=256, κ=cfg["kappa"], dt=cfg["dt"], T=T)),
+        ("turbulent2d_2048_etdrk4", W.turbulent_2d(2048),
+         lambda cfg, T: TAD.Problem(dev, ptf.SpectralStreamfunctionFlow(cfg["psih0"], cfg["omega"]),
+                                    nx=2048, κ=0.0, η=0.0, κh=cfg["kappah"], nκh=cfg["nkappah"],
+                                    dt=cfg["dt"], stepper="ETDRK4", T=T)),
+    )
+    for name, cfg, make in cases:
+        probs = {}
+        for T in (np.float64, np.float32):
+            p = make(cfg, T)
+            TAD.set_c_(p, cfg["c0"])
+            _tune(p, autotune)
+            ptf.stepforward_(p, cfg["nsteps"])
+            probs[T] = p
+        out[name] = dict(steps=cfg["nsteps"], **mixed_errors(TAD, probs[np.float32], probs[np.float64],
+                                                            cfg["c0"]))
+        del probs
+    return out
+
+
+def extra_forms(dev, steps, ens_steps, autotune="cache"):
     import passivetracerflows as ptf
     from passivetracerflows import workloads as W
     TAD = ptf.TracerAdvectionDiffusion
@@ -327,7 +392,7 @@ def extra_forms(dev, steps, ens_steps):
         p = TAD.Problem(dev, ptf.TwoDAdvectingFlow(u=lambda x, y: u, v=lambda x, y: v), nx=256,
                         κ=cfg["kappa"], dt=cfg["dt"], T=T)
         TAD.set_c_(p, cfg["c0"])
-        p.autotune()
+        _tune(p, autotune)
         ptf.stepforward_(p, 10)
         k = max(steps * 4, 200)
         s = time_steps(lambda n: ptf.stepforward_(p, n), k)
@@ -343,7 +408,7 @@ def extra_forms(dev, steps, ens_steps):
                         κ=0.0, η=0.0, κh=cfg["kappah"], nκh=cfg["nkappah"], dt=cfg["dt"],
                         stepper="ETDRK4", T=T)
         TAD.set_c_(p, cfg["c0"])
-        p.autotune()
+        _tune(p, autotune)
         ptf.stepforward_(p, 3)
         k = max(steps // 2, 20)
         s = time_steps(lambda n: ptf.stepforward_(p, n), k)
@@ -364,7 +429,7 @@ def extra_forms(dev, steps, ens_steps):
             e.set_flow(i, *m["flow"])
             e.set_c(i, m["c0"])
         if prec == "mixed":
-            e.autotune()
+            _tune(e, autotune)
         e.step(1, graph=True)  # graph capture + first replay outside the timed region
         k = min(ens_steps, ens_cfg["nsteps"]) - 1
         s = time_steps(lambda n: e.step(n, graph=True), k)
@@ -387,6 +452,9 @@ def main():
     ap.add_argument("--no-extra", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--retune", action="store_true",
+                    help="run the autotuner and overwrite its stored choices "
+                         "(default: apply the stored choices, tuning only what is missing)")
     ap.add_argument("--ensemble-steps", type=int, default=1000)
     ap.add_argument("--pmc-summary", nargs=5,
                     metavar=("FETCH_DIR", "WRITE_DIR", "CAL_FETCH_DIR", "CAL_WRITE_DIR", "BENCH_OUT"),
@@ -424,9 +492,9 @@ def main():
         return
     import passivetracerflows as ptf
     dev = ptf.ROCm(0)
+    autotune = False if args.no_autotune else ("retune" if args.retune else "cache")
 
-    cfg, prob, secs, (m0, m1, v0, v1) = headline(dev, args.steps, args.warmup,
-                                                 autotune=not args.no_autotune)
+    cfg, prob, secs, (m0, m1, v0, v1) = headline(dev, args.steps, args.warmup, autotune=autotune)
     s64 = prob.sol            # state after warmup + steps, for the mixed-form comparison
     reps, pcie_secs = repeat_and_pcie(prob, cfg, args.steps)
     l1 = float(np.abs(cfg["c0"]).sum())
@@ -439,16 +507,20 @@ def main():
     step_bytes = sum(r["bytes"] for r in rows if r["bytes"])
     step_ms_kernels = sum(r["ms"] for r in rows)
 
-    traffic = None
+    # measured HBM bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, scaled by a plain
+    # copy), used only when the measured kernel is the variant timed here
+    traffic, traffic_variant = None, None
     tpath = os.path.join(ROOT, "profiles", "traffic_abc3d_256_fp64.json")
     if os.path.exists(tpath):
         with open(tpath) as f:
             tj = json.load(f)
-        traffic = tj.get("per_launch_bytes", {}).get(dname)
+        ent = tj.get("families", {}).get(dname, {})
+        traffic_variant = ent.get("kernel")
+        if traffic_variant == kernel_template(fam["kernel"]):
+            traffic = ent.get("hbm_bytes_per_launch")
 
     # mixed form of the same workload + its error against the FP64 run
-    _, pm, secs32, _ = headline(dev, args.steps, args.warmup, T=np.float32,
-                                autotune=not args.no_autotune)
+    _, pm, secs32, _ = headline(dev, args.steps, args.warmup, T=np.float32, autotune=autotune)
     s32 = pm.sol
     mixed_err = float(np.linalg.norm(s32 - s64) / np.linalg.norm(s64))
     mixed_diag = dict(mass_err_rel_l1=abs(TAD_mass(pm) - m1) / float(np.abs(cfg["c0"]).sum()),
@@ -475,7 +547,8 @@ def main():
                                 steps_compared=args.steps + args.warmup),
     }
     if not args.no_extra:
-        forms.update(extra_forms(dev, args.steps, args.ensemble_steps))
+        forms.update(extra_forms(dev, args.steps, args.ensemble_steps, autotune))
+        forms["mixed_vs_fp64_full_length"] = full_length_mixed_errors(dev, autotune)
 
     cpu = None
     if not args.no_cpu:
@@ -499,7 +572,10 @@ def main():
                    "stepper": "RK4", "points": npts},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": f"{dname} [{fam['kernel']}]", "avg_launch_ms": avg_ms,
+                     "traffic_variant": traffic_variant,
+                     "kernel": f"{dname} [{fam['kernel']}]",
+                     "kernel_template": kernel_template(fam["kernel"]), "variants": prob.tuned,
+                     "avg_launch_ms": avg_ms,
                      "algorithmic_bytes_per_launch": avg_bytes},
         "cpu_baseline": cpu,
         "kernels": {k: dict(kernel=f["kernel"], ms_per_launch=f["total_ms"] / f["n"],

@@ -236,6 +236,11 @@ int ptf_ensemble_autotune(ptf_ensemble* ens);
 int ptf_ensemble_num_slots(ptf_ensemble* ens, int32_t* nslots);
 int ptf_ensemble_slot_name(ptf_ensemble* ens, int32_t slot, char* buf, int32_t buflen);
 int ptf_ensemble_time_slot(ptf_ensemble* ens, int32_t slot, int32_t reps, double* avg_ms);
+/* The ensemble's kernel variant of launch slot `slot` (as ptf_problem_set_variant /
+ * ptf_problem_get_variant): lets a caller store the autotuner's choice and apply it in
+ * another process without timing anything. */
+int ptf_ensemble_set_variant(ptf_ensemble* ens, int32_t slot, int32_t R, int32_t W, int32_t flags);
+int ptf_ensemble_get_variant(ptf_ensemble* ens, int32_t slot, int32_t* R, int32_t* W, int32_t* flags);
 
 #ifdef __cplusplus
 }

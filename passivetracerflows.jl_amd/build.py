@@ -30,6 +30,9 @@ COMMON = [
     "-I", CSRC, "-I", INCLUDE, "-Wno-unused-command-line-argument",
 ]
 KINDS = [0, 1, 2, 3, 4, 5, 6, 10, 11, 12, 13, 14]
+# parts per kind: the line lengths of the heavy kinds (stage epilogues, advection) are
+# dealt round-robin over several objects so that they compile in parallel (at most 6)
+PARTS = {3: 2, 4: 4, 5: 2, 6: 6, 12: 2, 13: 3, 14: 3}
 PRECS = [("double", "f64"), ("float", "f32")]
 HEADERS = ["ptf_device.hpp", "ptf_kernels.hpp"]
 API_HEADERS = HEADERS + ["ptf_setup.hpp"]
@@ -39,14 +42,27 @@ def _units():
     units = []
     for t, tn in PRECS:
         for k in KINDS:
-            obj = os.path.join(OBJDIR, f"reg_{tn}_{k}.o")
-            cmd = [HIPCC, *COMMON, f"-DPTF_T={t}", f"-DPTF_TNAME={tn}", f"-DPTF_KIND={k}",
-                   "-c", os.path.join(CSRC, "ptf_registry.hip"), "-o", obj]
-            units.append((obj, cmd, ["ptf_registry.hip", *HEADERS]))
+            nparts = PARTS.get(k, 1)
+            for part in range(nparts):
+                obj = os.path.join(OBJDIR, f"reg_{tn}_{k}_{part}.o")
+                cmd = [HIPCC, *COMMON, f"-DPTF_T={t}", f"-DPTF_TNAME={tn}", f"-DPTF_KIND={k}",
+                       f"-DPTF_PART={part}", f"-DPTF_NPARTS={nparts}",
+                       "-c", os.path.join(CSRC, "ptf_registry.hip"), "-o", obj]
+                units.append((obj, cmd, ["ptf_registry.hip", *HEADERS]))
+    # the heaviest units first, so that they do not end up on the critical path last
+    units.sort(key=lambda u: -_weight(u[0]))
     obj = os.path.join(OBJDIR, "ptf_api.o")
     units.append((obj, [HIPCC, *COMMON, "-c", os.path.join(CSRC, "ptf_api.hip"), "-o", obj],
                   ["ptf_api.hip", *API_HEADERS, os.path.join(INCLUDE, "ptf.h")]))
     return units
+
+
+def _weight(obj):
+    name = os.path.basename(obj)
+    if not name.startswith("reg_"):
+        return 0
+    k = int(name.split("_")[2])
+    return {6: 10, 4: 6, 12: 4, 13: 4, 14: 4, 3: 2, 5: 2}.get(k, 1) / PARTS.get(k, 1)
 
 
 def _stale(obj, deps):

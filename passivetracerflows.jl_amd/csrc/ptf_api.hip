@@ -8,7 +8,9 @@
 #include "ptf.h"
 #include "ptf_setup.hpp"
 
+#include <dlfcn.h>
 #include <hip/hip_runtime.h>
+#include <hip/hip_version.h>
 
 #include <algorithm>
 #include <cmath>
@@ -53,7 +55,13 @@ enum {
   K_ROW_C2R = 10, K_ROW_R2C = 11, K_ROW_ADV1 = 12, K_ROW_ADV2 = 13, K_ROW_ADV3 = 14
 };
 
-#define PTF_DECL(t, k) extern "C" const void* ptf_reg_##t##_##k(int, int, int);
+// Each (precision, kind) is compiled in up to kMaxParts parts (line lengths dealt
+// round-robin); a part that the build does not produce is a weak symbol left NULL.
+static constexpr int kMaxParts = 6;
+#define PTF_DECL1(t, k, p) extern "C" __attribute__((weak)) const void* ptf_reg_##t##_##k##_##p(int, int, int);
+#define PTF_DECL(t, k) \
+  PTF_DECL1(t, k, 0) PTF_DECL1(t, k, 1) PTF_DECL1(t, k, 2) PTF_DECL1(t, k, 3) PTF_DECL1(t, k, 4) \
+  PTF_DECL1(t, k, 5)
 #define PTF_DECL_ALL(t) \
   PTF_DECL(t, 0) PTF_DECL(t, 1) PTF_DECL(t, 2) PTF_DECL(t, 3) PTF_DECL(t, 4) PTF_DECL(t, 5) \
   PTF_DECL(t, 6) \
@@ -63,14 +71,42 @@ PTF_DECL_ALL(f32)
 
 typedef const void* (*RegFn)(int, int, int);
 
-static RegFn reg_fn(bool dbl, int kind) {
-#define PTF_CASE(k) case k: return dbl ? ptf_reg_f64_##k : ptf_reg_f32_##k;
-  switch (kind) {
-    PTF_CASE(0) PTF_CASE(1) PTF_CASE(2) PTF_CASE(3) PTF_CASE(4) PTF_CASE(5) PTF_CASE(6)
-    PTF_CASE(10) PTF_CASE(11) PTF_CASE(12) PTF_CASE(13) PTF_CASE(14)
-    default: return nullptr;
+// the registry function of one part (NULL when that part was not built)
+static RegFn reg_part(bool dbl, int kind, int part) {
+#define PTF_P(t, k) \
+  case k: { \
+    const RegFn f[kMaxParts] = {ptf_reg_##t##_##k##_0, ptf_reg_##t##_##k##_1, ptf_reg_##t##_##k##_2, \
+                                ptf_reg_##t##_##k##_3, ptf_reg_##t##_##k##_4, ptf_reg_##t##_##k##_5}; \
+    return f[part]; \
   }
-#undef PTF_CASE
+#define PTF_SW(t) \
+  switch (kind) { \
+    PTF_P(t, 0) PTF_P(t, 1) PTF_P(t, 2) PTF_P(t, 3) PTF_P(t, 4) PTF_P(t, 5) PTF_P(t, 6) \
+    PTF_P(t, 10) PTF_P(t, 11) PTF_P(t, 12) PTF_P(t, 13) PTF_P(t, 14) \
+    default: return nullptr; \
+  }
+  if (part < 0 || part >= kMaxParts) return nullptr;
+  if (dbl) {
+    PTF_SW(f64)
+  } else {
+    PTF_SW(f32)
+  }
+#undef PTF_SW
+#undef PTF_P
+}
+
+// the kernel of variant (N, R, W) of a kind, from whichever part holds line length N
+static const void* reg_lookup(bool dbl, int kind, int N, int R, int W) {
+  for (int part = 0; part < kMaxParts; ++part) {
+    const RegFn f = reg_part(dbl, kind, part);
+    if (!f) continue;
+    if (const void* p = f(N, R, W)) return p;
+  }
+  return nullptr;
+}
+
+static bool kind_known(int kind) {
+  return (kind >= 0 && kind <= 6) || (kind >= 10 && kind <= 14);
 }
 
 static const char* kind_name(int kind) {
@@ -135,8 +171,7 @@ struct Slot {
 };
 
 static bool choose_variant(Slot& s, int R, int W, int occ = 0, int par = 0) {
-  RegFn f = reg_fn(s.dbl, s.kind);
-  if (!f) return false;
+  if (!kind_known(s.kind)) return false;
   if (occ && !s.has_occ()) return false;
   if (par && !s.has_par()) return false;
   if (s.brick > 0) {
@@ -146,7 +181,7 @@ static bool choose_variant(Slot& s, int R, int W, int occ = 0, int par = 0) {
     if (s.is_col() && (R > s.N || (s.N / R) % (1 << s.brick) != 0)) return false;
     if (!s.is_col() && W % (1 << s.brick) != 0 && (1 << s.brick) % W != 0) return false;
   }
-  const void* p = f(s.N, R, W + 100 * par + 1000 * occ);
+  const void* p = reg_lookup(s.dbl, s.kind, s.N, R, W + 100 * par + 1000 * occ);
   if (!p) return false;
   s.R = R;
   s.W = W;
@@ -743,8 +778,11 @@ static int group_init(Group& g, const ptf_desc* descs, int count, bool allow_bri
 
 static int group_init_once(Group& g, const ptf_desc* descs, int count, int lb, bool plan_only);
 
+static int runtime_check();
+
 static int group_init(Group& g, const ptf_desc* descs, int count, bool allow_brick,
                       bool plan_only) {
+  CHK(runtime_check());
   const ptf_desc& d = descs[0];
   int lb = 0;
   if (allow_brick && d.ndim >= 2) {
@@ -1245,6 +1283,41 @@ static int clear_member_lo(Group& g, int member, hipStream_t st) {
   return PTF_OK;
 }
 
+// The library is compiled against one HIP runtime (HIP_VERSION_MAJOR.HIP_VERSION_MINOR).
+// A process that has already loaded another libamdhip64.so.7 (a framework that bundles
+// its own ROCm release) binds this library to that runtime as well, because the dynamic
+// loader reuses a loaded soname; code objects and runtime then come from different
+// releases, which has ended in host crashes.  Every entry point that first touches the
+// device checks the runtime it is bound to and refuses a different release.
+static const std::string& runtime_mismatch() {
+  static const std::string msg = [] {
+    int v = 0;
+    const hipError_t e = hipRuntimeGetVersion(&v);
+    Dl_info info;
+    std::memset(&info, 0, sizeof(info));
+    const char* path = "(unknown)";
+    if (dladdr(reinterpret_cast<void*>(&hipRuntimeGetVersion), &info) && info.dli_fname)
+      path = info.dli_fname;
+    const int major = v / 10000000, minor = (v / 100000) % 100;
+    if (e == hipSuccess && major == HIP_VERSION_MAJOR && minor == HIP_VERSION_MINOR)
+      return std::string();
+    char buf[512];
+    std::snprintf(buf, sizeof(buf),
+                  "HIP runtime mismatch: libptf_hip was built against HIP %d.%d but is bound to "
+                  "%s (HIP %d.%d, version %d). Another HIP runtime was loaded into this process "
+                  "first; load libptf_hip before any framework that bundles its own ROCm, or use "
+                  "a process without one",
+                  HIP_VERSION_MAJOR, HIP_VERSION_MINOR, path, major, minor, v);
+    return std::string(buf);
+  }();
+  return msg;
+}
+
+static int runtime_check() {
+  const std::string& m = runtime_mismatch();
+  return m.empty() ? PTF_OK : fail(PTF_ERR_DEVICE, m);
+}
+
 extern "C" {
 
 const char* ptf_last_error(void) { return g_err.c_str(); }
@@ -1256,6 +1329,7 @@ int ptf_version(int32_t* major, int32_t* minor) {
 }
 
 int ptf_device_init(int32_t ordinal) {
+  CHK(runtime_check());
   int n = 0;
   HIPCHK(hipGetDeviceCount(&n));
   if (ordinal < 0 || ordinal >= n)
@@ -1272,6 +1346,7 @@ int ptf_device_synchronize(void) {
 
 int ptf_buffer_alloc(uint64_t nbytes, void** dptr) {
   if (!dptr) return fail(PTF_ERR_INVALID, "dptr is NULL");
+  CHK(runtime_check());
   hipError_t e = hipMalloc(dptr, nbytes ? nbytes : 16);
   if (e != hipSuccess) {
     *dptr = nullptr;
@@ -2133,6 +2208,34 @@ int ptf_ensemble_time_slot(ptf_ensemble* ens, int32_t slot, int32_t reps, double
   auto v = ens_slots(ens);
   if (slot < 0 || slot >= (int)v.size()) return fail(PTF_ERR_INVALID, "slot out of range");
   return time_slot(*v[slot].first, *v[slot].second, reps, ens->stream, avg_ms);
+}
+
+int ptf_ensemble_set_variant(ptf_ensemble* ens, int32_t slot, int32_t R, int32_t W, int32_t flags) {
+  if (!ens) return fail(PTF_ERR_INVALID, "ensemble is NULL");
+  auto v = ens_slots(ens);
+  if (slot < 0 || slot >= (int)v.size()) return fail(PTF_ERR_INVALID, "slot out of range");
+  if (flags & ~(PTF_VARIANT_OCC3 | PTF_VARIANT_PAR)) return fail(PTF_ERR_INVALID, "unknown flags");
+  Slot trial = *v[slot].second;
+  if (!choose_variant(trial, R, W, (flags & PTF_VARIANT_OCC3) ? 3 : 0,
+                      (flags & PTF_VARIANT_PAR) ? 1 : 0))
+    return fail(PTF_ERR_UNSUPPORTED, "no compiled variant R=" + std::to_string(R) +
+                                         " W=" + std::to_string(W) + " flags=" +
+                                         std::to_string(flags) + " for slot " + trial.name);
+  *v[slot].second = trial;
+  drop_graph(ens);
+  return PTF_OK;
+}
+
+int ptf_ensemble_get_variant(ptf_ensemble* ens, int32_t slot, int32_t* R, int32_t* W,
+                             int32_t* flags) {
+  if (!ens || !R || !W || !flags) return fail(PTF_ERR_INVALID, "NULL argument");
+  auto v = ens_slots(ens);
+  if (slot < 0 || slot >= (int)v.size()) return fail(PTF_ERR_INVALID, "slot out of range");
+  const Slot& s = *v[slot].second;
+  *R = s.R;
+  *W = s.W;
+  *flags = (s.occ ? PTF_VARIANT_OCC3 : 0) | (s.par ? PTF_VARIANT_PAR : 0);
+  return PTF_OK;
 }
 
 }  // extern "C"

@@ -180,10 +180,8 @@ void col_kernel(ColArgs<T> a) {
     const cpx<T>* in0 = a.in0 + ubase;
 #pragma unroll
     for (int q = 0; q < R; ++q) {
-      // the 3D y-inverse pass reads its inputs once (unless it re-reads in0 below for
-      // want of registers): non-temporal loads there
-      constexpr bool kNt = MODE == COL_D2 && (sizeof(T) == 4 || R <= 8);
-      if constexpr (kNt) v[q] = ld_nt(in0 + off(q));
+      // the 3D y-inverse pass reads its inputs once: non-temporal loads there
+      if constexpr (MODE == COL_D2) v[q] = ld_nt(in0 + off(q));
       else v[q] = in0[off(q)];
     }
 
@@ -196,19 +194,18 @@ void col_kernel(ColArgs<T> a) {
       }
     } else if constexpr (MODE == COL_D1 || MODE == COL_D2) {
       // The derivative is transformed and stored first, because out0 may alias in0
-      // (in place).  When registers allow, the plain copy (and in1) stay in
-      // registers from the first load instead of being re-read after a store.
+      // (in place).  The plain copy stays in registers from the first load, so the
+      // input is read from memory once.  When registers allow (not FP64 at radix 16),
+      // in1 is loaded together with in0, otherwise after the second transform.
       constexpr bool kKeep = sizeof(T) == 4 || R <= 8;
-      cpx<T> u[kKeep ? R : 1];
+      cpx<T> u[R];
       cpx<T> p2[(MODE == COL_D2 && kKeep) ? R : 1];
-      if constexpr (kKeep) {
 #pragma unroll
-        for (int q = 0; q < R; ++q) u[q] = v[q];
-        if constexpr (MODE == COL_D2) {
-          const cpx<T>* in1 = a.in1 + ubase;
+      for (int q = 0; q < R; ++q) u[q] = v[q];
+      if constexpr (kKeep && MODE == COL_D2) {
+        const cpx<T>* in1 = a.in1 + ubase;
 #pragma unroll
-          for (int q = 0; q < R; ++q) p2[q] = ld_nt(in1 + off(q));
-        }
+        for (int q = 0; q < R; ++q) p2[q] = ld_nt(in1 + off(q));
       }
 #pragma unroll
       for (int q = 0; q < R; ++q) {
@@ -224,10 +221,7 @@ void col_kernel(ColArgs<T> a) {
         }
       }
 #pragma unroll
-      for (int q = 0; q < R; ++q) {
-        if constexpr (kKeep) v[q] = u[q];
-        else v[q] = in0[off(q)];
-      }
+      for (int q = 0; q < R; ++q) v[q] = u[q];
       fft(std::integral_constant<int, 1>{}, v);
       {
         cpx<T>* o = a.out0 + ubase;

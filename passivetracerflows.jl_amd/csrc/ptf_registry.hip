@@ -1,7 +1,9 @@
-// Kernel-variant registry.  This file is compiled once per (precision, kind)
-// pair, with -DPTF_T=<double|float> -DPTF_TNAME=<f64|f32> -DPTF_KIND=<k>, and
-// exports  const void* ptf_reg_<TNAME>_<KIND>(int N, int R, int W)  returning
-// the kernel for line length N (column kinds: axis length; row kinds: nx),
+// Kernel-variant registry.  This file is compiled once per (precision, kernel kind,
+// part), with -DPTF_T=<double|float> -DPTF_TNAME=<f64|f32> -DPTF_KIND=<k>
+// -DPTF_PART=<p> -DPTF_NPARTS=<n> (the line lengths of a kind are dealt round-robin
+// over its n parts so that the heavy kinds compile in parallel), and exports
+//   const void* ptf_reg_<TNAME>_<KIND>_<PART>(int N, int R, int W)
+// returning the kernel for line length N (column kinds: axis length; row kinds: nx),
 // radix R and width W (columns per block TX, or rows per block NL; plus
 // 100 for the input-parallel advection form and 1000 * occ for the variants
 // built with a 3-waves/SIMD budget), or NULL
@@ -13,6 +15,12 @@ using namespace ptf;
 
 #ifndef PTF_T
 #error "PTF_T must be defined"
+#endif
+#ifndef PTF_PART
+#define PTF_PART 0
+#endif
+#ifndef PTF_NPARTS
+#define PTF_NPARTS 1
 #endif
 
 namespace {
@@ -156,32 +164,39 @@ const void* by_r(int R, int W) {
   }
 }
 
+// line length N, the I-th of its kind's list: compiled into part I mod PTF_NPARTS
+template <typename T, int N, int KIND, int I>
+const void* in_part(int R, int W) {
+  if constexpr (I % PTF_NPARTS == PTF_PART) return by_r<T, N, KIND>(R, W);
+  else return nullptr;
+}
+
 template <typename T, int KIND>
 const void* by_n(int N, int R, int W) {
   if constexpr (KIND < 10) {
     switch (N) {
-      case 8: return by_r<T, 8, KIND>(R, W);
-      case 16: return by_r<T, 16, KIND>(R, W);
-      case 32: return by_r<T, 32, KIND>(R, W);
-      case 64: return by_r<T, 64, KIND>(R, W);
-      case 128: return by_r<T, 128, KIND>(R, W);
-      case 256: return by_r<T, 256, KIND>(R, W);
-      case 512: return by_r<T, 512, KIND>(R, W);
-      case 1024: return by_r<T, 1024, KIND>(R, W);
-      case 2048: return by_r<T, 2048, KIND>(R, W);
+      case 8: return in_part<T, 8, KIND, 0>(R, W);
+      case 16: return in_part<T, 16, KIND, 1>(R, W);
+      case 32: return in_part<T, 32, KIND, 2>(R, W);
+      case 64: return in_part<T, 64, KIND, 3>(R, W);
+      case 128: return in_part<T, 128, KIND, 4>(R, W);
+      case 256: return in_part<T, 256, KIND, 5>(R, W);
+      case 512: return in_part<T, 512, KIND, 6>(R, W);
+      case 1024: return in_part<T, 1024, KIND, 7>(R, W);
+      case 2048: return in_part<T, 2048, KIND, 8>(R, W);
       default: return nullptr;
     }
   } else {
     switch (N) {  // nx
-      case 16: return by_r<T, 16, KIND>(R, W);
-      case 32: return by_r<T, 32, KIND>(R, W);
-      case 64: return by_r<T, 64, KIND>(R, W);
-      case 128: return by_r<T, 128, KIND>(R, W);
-      case 256: return by_r<T, 256, KIND>(R, W);
-      case 512: return by_r<T, 512, KIND>(R, W);
-      case 1024: return by_r<T, 1024, KIND>(R, W);
-      case 2048: return by_r<T, 2048, KIND>(R, W);
-      case 4096: return by_r<T, 4096, KIND>(R, W);
+      case 16: return in_part<T, 16, KIND, 0>(R, W);
+      case 32: return in_part<T, 32, KIND, 1>(R, W);
+      case 64: return in_part<T, 64, KIND, 2>(R, W);
+      case 128: return in_part<T, 128, KIND, 3>(R, W);
+      case 256: return in_part<T, 256, KIND, 4>(R, W);
+      case 512: return in_part<T, 512, KIND, 5>(R, W);
+      case 1024: return in_part<T, 1024, KIND, 6>(R, W);
+      case 2048: return in_part<T, 2048, KIND, 7>(R, W);
+      case 4096: return in_part<T, 4096, KIND, 8>(R, W);
       default: return nullptr;
     }
   }
@@ -189,9 +204,9 @@ const void* by_n(int N, int R, int W) {
 
 }  // namespace
 
-#define PTF_JOIN3_(a, b, c) a##_##b##_##c
-#define PTF_JOIN3(a, b, c) PTF_JOIN3_(a, b, c)
+#define PTF_JOIN4_(a, b, c, d) a##_##b##_##c##_##d
+#define PTF_JOIN4(a, b, c, d) PTF_JOIN4_(a, b, c, d)
 
-extern "C" const void* PTF_JOIN3(ptf_reg, PTF_TNAME, PTF_KIND)(int N, int R, int W) {
+extern "C" const void* PTF_JOIN4(ptf_reg, PTF_TNAME, PTF_KIND, PTF_PART)(int N, int R, int W) {
   return by_n<PTF_T, PTF_KIND>(N, R, W);
 }

@@ -124,6 +124,8 @@ SIGNATURES = {
     "ptf_ensemble_num_slots": [_VP, C.POINTER(_I32)],
     "ptf_ensemble_slot_name": [_VP, _I32, C.c_char_p, _I32],
     "ptf_ensemble_time_slot": [_VP, _I32, _I32, _DP],
+    "ptf_ensemble_set_variant": [_VP, _I32, _I32, _I32, _I32],
+    "ptf_ensemble_get_variant": [_VP, _I32, C.POINTER(_I32), C.POINTER(_I32), C.POINTER(_I32)],
 }
 
 _lib = None

@@ -38,6 +38,8 @@ class Ensemble:
             d.dt = float(dt)
             d.flow_kind = _lib.PTF_FLOW_STEADY
             self.sizes.append(n)
+        self.precision = "mixed" if precision == "mixed" else "fp64"
+        self.stepper = stepper.upper()
         h = C.c_void_p()
         call("ptf_ensemble_create", arr, len(members), C.byref(h))
         self._h = h.value
@@ -72,8 +74,28 @@ class Ensemble:
     def set_stream(self, hip_stream):
         call("ptf_ensemble_set_stream", self._h, hip_stream)
 
-    def autotune(self):
-        call("ptf_ensemble_autotune", self._h)
+    def autotune(self, cache=None):
+        """Fastest compiled variant of every launch slot; with ``cache`` (a JSON path) the
+        choice is stored there and reused by later runs (see passivetracerflows.tuning)."""
+        if cache is None:
+            call("ptf_ensemble_autotune", self._h)
+            return "tuned"
+        from . import tuning
+        return tuning.apply_or_tune(self, self.tuning_key(), cache,
+                                    tune=lambda: call("ptf_ensemble_autotune", self._h))
+
+    def tuning_key(self):
+        from collections import Counter
+        sizes = ",".join(f"{c}x{n}^2" for n, c in sorted(Counter(self.sizes).items()))
+        return f"ensemble {sizes} {self.precision} {self.stepper}"
+
+    def set_variant(self, i, R, W, occ3=False, par=False):
+        call("ptf_ensemble_set_variant", self._h, i, R, W, (1 if occ3 else 0) | (2 if par else 0))
+
+    def get_variant(self, i):
+        R, W, f = C.c_int32(), C.c_int32(), C.c_int32()
+        call("ptf_ensemble_get_variant", self._h, i, C.byref(R), C.byref(W), C.byref(f))
+        return R.value, W.value, bool(f.value & 1), bool(f.value & 2)
 
     def slots(self):
         n = C.c_int32()

@@ -187,12 +187,23 @@ class DeviceArray:
             pass
 
 
-def _devptr(a):
-    if isinstance(a, DeviceArray):
-        return a.ptr
-    if hasattr(a, "data_ptr"):  # torch tensor on the GPU
-        return a.data_ptr()
-    raise TypeError("expected a DeviceArray or a device tensor")
+def _devptr(a, shape=None, dtype=None, what="array"):
+    """Device address of a DeviceArray after checking it against what the plan expects.
+
+    Only library-owned DeviceArrays are accepted: the kernels index a buffer by the
+    plan's sizes, so a too-small, strided or wrongly typed buffer would be read and
+    written out of bounds.  (Foreign device tensors are refused for the same reason,
+    and because a framework that brings its own HIP runtime into the process makes
+    the library refuse to start.)"""
+    if not isinstance(a, DeviceArray):
+        raise TypeError(f"{what}: expected a DeviceArray (a libptf_hip buffer), got {type(a).__name__}")
+    if not a.ptr:
+        raise ValueError(f"{what}: the DeviceArray has been freed")
+    if shape is not None and tuple(a.shape) != tuple(shape):
+        raise ValueError(f"{what}: shape {tuple(a.shape)} does not match the plan's {tuple(shape)}")
+    if dtype is not None and a.dtype != np.dtype(dtype):
+        raise ValueError(f"{what}: dtype {a.dtype} does not match the plan's {np.dtype(dtype)}")
+    return a.ptr
 
 
 class RFFTPlan:
@@ -223,15 +234,34 @@ class RFFTPlan:
             pass
 
 
+def _plan_shapes(plan: RFFTPlan):
+    """(physical shape, spectral shape, real dtype, complex dtype) of the plan's arrays."""
+    g = plan.grid
+    lead = (plan.nlayers,) if plan.nlayers > 1 else ()
+    rt = np.dtype(g.T)
+    ct = np.dtype(np.complex128 if rt == np.float64 else np.complex64)
+    return lead + tuple(g.shape), lead + tuple(g.spec_shape), rt, ct
+
+
 def mul_(out, plan: RFFTPlan, a):
-    """mul!(out, rfftplan, a): unnormalised real-to-complex transform (device arrays)."""
-    call("ptf_rfft", plan._h, _devptr(a), _devptr(out))
+    """mul!(out, rfftplan, a): unnormalised real-to-complex transform (device arrays).
+
+    Like FFTW's mul!, the arrays must have the plan's sizes (and its precision):
+    ValueError otherwise, before anything is launched."""
+    rs, ss, rt, ct = _plan_shapes(plan)
+    pin = _devptr(a, rs, rt, "input")
+    pout = _devptr(out, ss, ct, "output")
+    call("ptf_rfft", plan._h, pin, pout)
     return out
 
 
 def ldiv_(out, plan: RFFTPlan, a):
-    """ldiv!(out, rfftplan, a): complex-to-real inverse scaled by 1/(nx ny nz); `a` is preserved."""
-    call("ptf_irfft", plan._h, _devptr(a), _devptr(out))
+    """ldiv!(out, rfftplan, a): complex-to-real inverse scaled by 1/(nx ny nz); `a` is preserved.
+    Shapes and precision are checked as in mul_."""
+    rs, ss, rt, ct = _plan_shapes(plan)
+    pin = _devptr(a, ss, ct, "input")
+    pout = _devptr(out, rs, rt, "output")
+    call("ptf_irfft", plan._h, pin, pout)
     return out
 
 
@@ -240,7 +270,8 @@ def dealias_(fh, grid, plan: RFFTPlan | None = None):
     array; grid.aliased_fraction, default 1/3 = the 2/3 rule).  Pass the layered plan
     for an (nlayers, ...) array."""
     plan = grid.rfftplan if plan is None else plan
-    call("ptf_dealias", plan._h, _devptr(fh), float(grid.aliased_fraction))
+    _, ss, _, ct = _plan_shapes(plan)
+    call("ptf_dealias", plan._h, _devptr(fh, ss, ct, "fh"), float(grid.aliased_fraction))
     call("ptf_device_synchronize")
     return None
 
@@ -269,6 +300,15 @@ class Equation:
         self.params = params
         self.dims = L.shape
         self.T = np.complex128 if grid.T is np.float64 else np.complex64
+
+
+def tuning_key(prob):
+    """Identity of a problem's launch list for the tuning cache."""
+    g = prob.grid
+    from . import traceradvectiondiffusion as tad
+    prec = "fp64" if g.T is np.float64 else "mixed"
+    return (f"{g.ndim}D {g.nx}x{g.ny}x{g.nz} {prec} {prob.timestepper} "
+            f"flow{tad._flow_kind(prob.params)}")
 
 
 class Problem:
@@ -330,8 +370,18 @@ class Problem:
     def set_stream(self, hip_stream):
         call("ptf_problem_set_stream", self._h, hip_stream)
 
-    def autotune(self):
-        call("ptf_autotune", self._h)
+    def autotune(self, cache=None):
+        """Pick the fastest compiled kernel variant of every launch slot (ptf_autotune).
+
+        With ``cache`` (a JSON file path) the choice is persistent: if the file holds an
+        entry for this problem's shape, precision, stepper and flow kind whose slot list
+        matches, those variants are applied without timing anything; otherwise the
+        autotuner runs and its choice is stored there.  Returns "cache" or "tuned"."""
+        if cache is None:
+            call("ptf_autotune", self._h)
+            return "tuned"
+        from passivetracerflows import tuning
+        return tuning.apply_or_tune(self, tuning_key(self), cache)
 
     def set_graph(self, enable: bool):
         """Replay captured 8-step graphs for steady flows (default) or launch directly."""

@@ -1,0 +1,50 @@
+"""Persistent kernel-variant choices (the autotuner's output, reused across processes).
+
+The autotuner (``ptf_autotune``) times every compiled variant of every launch slot on the
+problem's own buffers and keeps the fastest.  Its choice varies a little from run to run
+where two variants are within noise of each other, so a benchmark that is profiled in a
+second process would otherwise time one variant and profile another.  A cache file maps a
+problem key (shape, precision, stepper, flow kind) to the per-slot (radix, width,
+3-waves/SIMD, input-parallel) choice; applying it needs no timing.
+"""
+from __future__ import annotations
+
+import json
+import os
+
+
+def _base(name: str) -> str:
+    return name.split(" [")[0]
+
+
+def load(path):
+    if path and os.path.exists(path):
+        with open(path) as f:
+            return json.load(f)
+    return {}
+
+
+def store(path, data):
+    tmp = path + ".tmp"
+    with open(tmp, "w") as f:
+        json.dump(data, f, indent=1, sort_keys=True)
+        f.write("\n")
+    os.replace(tmp, path)
+
+
+def apply_or_tune(obj, key, path, tune=None):
+    """Apply the cached variants of ``key`` to ``obj`` (a Problem or an Ensemble: slots(),
+    set_variant(i, R, W, occ3, par), get_variant(i)); if there is no matching entry, run the
+    autotuner (``tune``, default the object's own) and store its choice.  Returns "cache" or
+    "tuned"."""
+    data = load(path)
+    names = [_base(n) for n in obj.slots()]
+    entry = data.get(key)
+    if entry and [e[0] for e in entry] == names:
+        for i, (_, R, W, occ3, par) in enumerate(entry):
+            obj.set_variant(i, int(R), int(W), occ3=bool(occ3), par=bool(par))
+        return "cache"
+    (tune or obj.autotune)()
+    data[key] = [[names[i], *obj.get_variant(i)] for i in range(len(names))]
+    store(path, data)
+    return "tuned"

@@ -56,6 +56,10 @@ CASES = [
     # long lines (>= 512): one-line-per-block column variants and part-brick row blocks
     (2, (64, 512), "RK4", np.float64), (2, (128, 512), "ETDRK4", np.float32),
     (3, (16, 512, 64), "RK4", np.float32), (3, (16, 16, 512), "ETDRK4", np.float64),
+    # 1024- and 2048-point lines: twiddle tables in LDS at their largest, and the
+    # (width, radix) variants that still fit the 160 KiB LDS budget
+    (2, (2048, 64), "RK4", np.float64), (2, (64, 2048), "ETDRK4", np.float32),
+    (3, (16, 16, 1024), "RK4", np.float64), (2, (1024, 64), "ETDRK4", np.float32),
 ]
 
 
@@ -63,8 +67,9 @@ CASES = [
 def test_every_variant_of_every_slot(dev, ndim, n, stepper, T):
     print(f"variants ndim={ndim} n={n} {stepper} {T.__name__}", flush=True)
     g, flows, c0 = _setup(ndim, n, seed=11 + ndim)
-    # kappa |k_max|^2 dt stays inside RK4's real-axis stability limit for lines of 512
-    dt, kappa, nsteps = 0.01, (1e-3 if max(n) >= 512 else 1e-2), 3
+    # kappa |k_max|^2 dt stays inside RK4's real-axis stability limit for lines of 512..2048
+    kappa = 1e-4 if max(n) >= 1024 else (1e-3 if max(n) >= 512 else 1e-2)
+    dt, nsteps = 0.01, 3
     o = O.Problem(g, flows, kappa=kappa, dt=dt, stepper=stepper)
     o.set_c(c0)
     sol0 = o.sol.copy()
