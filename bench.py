@@ -42,10 +42,11 @@ HBM_PEAK_GBS = 8000.0
 PASS_MODEL_B_PER_GPS = 716.8
 
 
-def kernel_bytes(name: str, nS: int, nR: int, cb: int, rb: int, stage: int):
+def kernel_bytes(name: str, nS: int, nR: int, cb: int, rb: int, stage: int, stepper: str = "RK4"):
     """Algorithmic HBM bytes of one launch (pass model): every field a kernel must
     read or write once.  cb/rb = bytes per complex/real element."""
     S, R = nS * cb, nR * rb
+    C = nS * 8       # an FP64 ETDRK4 coefficient array (S-shaped, real)
     fam = name.split(" [")[0].split("_", 1)[1] if name.startswith("stage") else name.split(" [")[0]
     if fam.startswith("prologue_"):
         return 3 * S
@@ -63,8 +64,14 @@ def kernel_bytes(name: str, nS: int, nR: int, cb: int, rb: int, stage: int):
     if fam == "y_forward":
         return 2 * S
     # RK4 state traffic of the FP64 form: stage 0 reads sol, writes acc; 1 reads sol, acc,
-    # writes stg, acc; 2 reads stg, sol, acc, writes stg, sol; 3 reads stg, sol, writes sol
-    state = {0: 2 * S, 1: 4 * S, 2: 5 * S, 3: 3 * S}[stage]
+    # writes stg, acc; 2 reads stg, sol, acc, writes stg, sol; 3 reads stg, sol, writes sol.
+    # ETDRK4: 0 and 1 read sol, E2, zeta and write the next input and N; 2 reads sol, sol1,
+    # N1, N2, E2, zeta, E, alpha, beta and writes sol2 and the partial update; 3 reads the
+    # partial update and Gamma and writes sol.
+    if stepper.upper().endswith("ETDRK4"):
+        state = {0: 3 * S + 2 * C, 1: 3 * S + 2 * C, 2: 6 * S + 5 * C, 3: 2 * S + C}[stage]
+    else:
+        state = {0: 2 * S, 1: 4 * S, 2: 5 * S, 3: 3 * S}[stage]
     if fam in ("z_forward_stage", "y_forward_stage"):
         return S + state
     if fam in ("z_forward_stage_inverse", "y_forward_stage_inverse"):
@@ -72,6 +79,10 @@ def kernel_bytes(name: str, nS: int, nR: int, cb: int, rb: int, stage: int):
         return S + state + 2 * S
     if fam in ("prologue_z_inverse_deriv", "prologue_y_inverse_deriv"):
         return 3 * S
+    if fam == "flow_y_synthesis":          # psi0 (FP64 complex) and omega (FP64) in, 2 out
+        return nS * 16 + nS * 8 + 2 * S
+    if fam.startswith("flow_x_c2r"):
+        return S + R
     return None
 
 
@@ -174,6 +185,59 @@ def trace_summary(trace_dir, bench_out, steps, warmup):
     return "\n".join(lines)
 
 
+def ensemble_summary(trace_dir, fetch_dir, write_dir, cal_fetch_dir, cal_write_dir, bench_out):
+    """Markdown summary of rocprofv3 runs of `bench.py --profile-form ensemble_*`: per kernel
+    template, dispatches per ensemble step, average duration (kernel trace) and HBM bytes
+    per dispatch (FETCH_SIZE and WRITE_SIZE scaled by the plain-copy calibration), over the
+    last steps of the run (the timed ones)."""
+    import csv
+    import glob
+    bj = json.loads(open(bench_out).read().strip().splitlines()[-1])
+    steps = bj["steps"]
+    disp = []
+    for path in glob.glob(os.path.join(trace_dir, "**", "*kernel_trace.csv"), recursive=True):
+        with open(path, newline="") as f:
+            for r in csv.DictReader(f):
+                disp.append((int(r["Start_Timestamp"]), int(r["End_Timestamp"]), r["Kernel_Name"]))
+    disp.sort()
+    kern = [d for d in disp if "kernel<" in d[2]]
+    # dispatches of one ensemble step: the step kernels of the last `steps` steps
+    names = sorted({d[2].split("(")[0] for d in kern})
+    per_step = {}
+    for n in names:
+        ds = [e - b for (b, e, k) in kern if k.split("(")[0] == n]
+        per_step[n] = ds
+    calib = {}
+    for counter, d in (("FETCH_SIZE", cal_fetch_dir), ("WRITE_SIZE", cal_write_dir)):
+        v = [x for (k, c, x) in _counter_rows(d) if c == counter and "copyBuffer" in k]
+        calib[counter] = (1 << 30) / (sum(v) / len(v) * 1024.0) if v else None
+    pmc = {}
+    for counter, d in (("FETCH_SIZE", fetch_dir), ("WRITE_SIZE", write_dir)):
+        for (k, c, x) in _counter_rows(d):
+            if c == counter:
+                pmc.setdefault((k.split("(")[0], counter), []).append(x)
+    lines = [f"ensemble step: {bj['ms_per_step']:.4f} ms ({bj['value']:.4g} grid-point-steps/s, "
+             f"{bj['points_per_step']} points per step)", "",
+             "| kernel | dispatches (run) | avg us (last dispatches) | HBM MB / dispatch | GB/s |",
+             "|---|---|---|---|---|"]
+    tot_us = 0.0
+    for n in names:
+        ds = per_step[n]
+        last = ds[-max(1, len(ds) // 2):]
+        avg = sum(last) / len(last) / 1e3
+        mb = None
+        if calib["FETCH_SIZE"] and calib["WRITE_SIZE"]:
+            f = pmc.get((n, "FETCH_SIZE"))
+            w = pmc.get((n, "WRITE_SIZE"))
+            if f and w:
+                mb = (sum(f) / len(f) * 1024 * calib["FETCH_SIZE"] +
+                      sum(w) / len(w) * 1024 * calib["WRITE_SIZE"]) / 1e6
+        lines.append(f"| `{n}` | {len(ds)} | {avg:.1f} | {mb:.1f} | {mb / avg * 1e3:.0f} |"
+                     if mb else f"| `{n}` | {len(ds)} | {avg:.1f} | - | - |")
+        tot_us += avg
+    return "\n".join(lines)
+
+
 def TAD_mass(p):
     import passivetracerflows as ptf
     return ptf.TracerAdvectionDiffusion.mass(p)
@@ -259,13 +323,13 @@ def repeat_and_pcie(prob, cfg, steps):
     return reps, pcie
 
 
-def slot_table(prob, nS, nR, cb, rb, steps):
+def slot_table(prob, nS, nR, cb, rb, steps, stepper="RK4"):
     """Per-launch durations measured inside `steps` running steps (a HIP event around
     every launch on the problem's stream), with each launch's algorithmic bytes."""
     rows = []
     for i, (name, ms) in enumerate(zip(prob.slots(), prob.step_profiled(steps))):
         stage = int(name[5]) - 1 if name.startswith("stage") else 0
-        b = kernel_bytes(name, nS, nR, cb, rb, stage)
+        b = kernel_bytes(name, nS, nR, cb, rb, stage, stepper)
         rows.append(dict(slot=i, name=name, ms=ms, bytes=b,
                          gbs=(b / (ms * 1e-3) / 1e9) if b else None))
     return rows
@@ -316,23 +380,35 @@ def cpu_baseline(cfg, steps):
                        f"warm-up step, {dt:.1f} s")
 
 
-def ensemble_parity(members, e32, e64, dt, steps):
+def ensemble_parity(members, e32, e64, e64r, dt, steps):
     """Every member's FP32 state against its own FP64 run after `steps` steps, with the
-    members classed by workloads.member_class (RK4 diffusion limit, spectral resolution)."""
+    FP64 problem's own sensitivity as the control: e64r is the FP64 ensemble run from the
+    inputs (c0, flow) rounded to FP32, and spread = relL2(e64r, e64) per member.  Members
+    are classed by workloads.member_class (RK4 diffusion limit, spectral resolution);
+    `diverged` are members whose FP64 run leaves the FP32 range."""
     from passivetracerflows import workloads as W
     classes = {"resolved": 0, "under_resolved": 0, "diffusion_unstable": 0}
-    worst_resolved, within = 0.0, 0
+    worst_resolved, within, within_ctrl, diverged = 0.0, 0, 0, 0
+    ratio_resolved = 0.0
     for i, m in enumerate(members):
         a, b = e32.get_sol(i), e64.get_sol(i)
-        nb = float(np.linalg.norm(b))
-        r = float(np.linalg.norm(a - b) / nb) if np.isfinite(nb) and nb > 0 else float("nan")
         cls = W.member_class(m, dt, b)
+        classes[cls] += 1
+        with np.errstate(all="ignore"):
+            if not (np.all(np.isfinite(b)) and np.abs(b).max() < 1e30):
+                diverged += 1
+                continue
+            r = float(np.linalg.norm(a - b) / np.linalg.norm(b))
+            sp = float(np.linalg.norm(e64r.get_sol(i) - b) / np.linalg.norm(b))
         if cls == "resolved":
             worst_resolved = max(worst_resolved, r)
-        classes[cls] += 1
-        within += int(np.isfinite(r) and r <= 1e-5)
-    return dict(steps=steps, members=len(members), classes=classes,
-                max_relL2_resolved=worst_resolved, members_within_1e5=within)
+            ratio_resolved = max(ratio_resolved, r / sp if sp > 0 else 0.0)
+        within += int(r <= 1e-5)
+        within_ctrl += int(r <= max(1e-5, 300 * sp))
+    return dict(steps=steps, members=len(members), classes=classes, diverged=diverged,
+                max_relL2_resolved=worst_resolved, members_within_1e5=within,
+                max_error_over_spread_resolved=ratio_resolved,
+                members_within_max_1e5_300_spread=within_ctrl)
 
 
 def mixed_errors(TAD, p32, p64, c0):
@@ -436,11 +512,76 @@ def extra_forms(dev, steps, ens_steps, autotune="cache"):
         res[prec] = (e, s, k)
     e32, s32, k = res["mixed"]
     e64, s64, _ = res["fp64"]
+    # the control: FP64 from inputs rounded to FP32 (not timed)
+    r32 = lambda a: np.asarray(a, np.float32).astype(np.float64)
+    e64r = ptf.Ensemble([dict(n=m["n"], kappa=m["kappa"]) for m in members], dt=ens_cfg["dt"],
+                        precision="fp64")
+    for i, m in enumerate(members):
+        e64r.set_flow(i, *[r32(f) for f in m["flow"]])
+        e64r.set_c(i, r32(m["c0"]))
+    e64r.step(k + 1, graph=True)
     out["ensemble512_fp32_graph"] = dict(value=gps(npts, k, s32), ms_per_step=1e3 * s32 / k, steps=k,
                                          points_per_step=npts)
     out["ensemble512_fp64_graph"] = dict(value=gps(npts, k, s64), ms_per_step=1e3 * s64 / k, steps=k)
-    out["ensemble512_parity"] = ensemble_parity(members, e32, e64, ens_cfg["dt"], k + 1)
+    out["ensemble512_parity"] = ensemble_parity(members, e32, e64, e64r, ens_cfg["dt"], k + 1)
     return out
+
+
+PROFILE_FORMS = ("abc3d_fp64", "abc3d_mixed", "turb2d_fp64", "turb2d_mixed", "ensemble_fp32",
+                 "ensemble_fp64")
+
+
+def profile_form(dev, form, steps, warmup, autotune):
+    """One workload alone (for a rocprofv3 run): stored variants, `warmup` steps, `steps`
+    timed steps, then (single problems) `steps` steps with an event around every launch.
+    Prints one JSON line: ms/step, GPS/s and, per kernel family, the variant, its
+    average in-step duration and its algorithmic bytes."""
+    import passivetracerflows as ptf
+    from passivetracerflows import workloads as W
+    TAD = ptf.TracerAdvectionDiffusion
+    out = dict(form=form, steps=steps, warmup=warmup)
+    if form.startswith("ensemble"):
+        cfg = W.ensemble_2d()
+        members = [dict(m, **W.ensemble_member(m["n"], m["seed"])) for m in cfg["members"]]
+        prec = "mixed" if form.endswith("fp32") else "fp64"
+        e = ptf.Ensemble([dict(n=m["n"], kappa=m["kappa"]) for m in members], dt=cfg["dt"],
+                         precision=prec)
+        for i, m in enumerate(members):
+            e.set_flow(i, *m["flow"])
+            e.set_c(i, m["c0"])
+        out["variants"] = _tune(e, autotune)
+        e.step(max(warmup, 1), graph=True)
+        secs = time_steps(lambda n: e.step(n, graph=True), steps)
+        npts = sum(m["n"] ** 2 for m in members)
+        out.update(ms_per_step=1e3 * secs / steps, value=gps(npts, steps, secs),
+                   points_per_step=npts, slots=e.slots())
+        print(json.dumps(out), flush=True)
+        return
+    T = np.float64 if form.endswith("fp64") else np.float32
+    if form.startswith("abc3d"):
+        cfg, prob, secs, _ = headline(dev, steps, warmup, T=T, autotune=autotune)
+        n, stepper = (256, 256, 256), "RK4"
+    else:
+        cfg = W.turbulent_2d(2048)
+        prob = TAD.Problem(dev, ptf.SpectralStreamfunctionFlow(cfg["psih0"], cfg["omega"]), nx=2048,
+                           κ=0.0, η=0.0, κh=cfg["kappah"], nκh=cfg["nkappah"], dt=cfg["dt"],
+                           stepper="ETDRK4", T=T)
+        TAD.set_c_(prob, cfg["c0"])
+        prob.tuned = _tune(prob, autotune)
+        ptf.stepforward_(prob, warmup)
+        secs = time_steps(lambda k: ptf.stepforward_(prob, k), steps)
+        n, stepper = (2048, 2048, 1), "ETDRK4"
+    npts = n[0] * n[1] * n[2]
+    nS = (n[0] // 2 + 1) * n[1] * n[2]
+    cb, rb = (16, 8) if T is np.float64 else (8, 4)
+    rows = slot_table(prob, nS, npts, cb, rb, steps, stepper)
+    _, _, _, _, fams = dominant(rows)
+    out.update(ms_per_step=1e3 * secs / steps, value=gps(npts, steps, secs), variants=prob.tuned,
+               kernels={k: dict(kernel=f["kernel"], ms_per_launch=f["total_ms"] / f["n"],
+                                launches_per_step=f["n"], bytes_per_launch=f["bytes"] / f["n"],
+                                GBs=(f["bytes"] / f["total_ms"] * 1e3 / 1e9) if f["bytes"] else None)
+                        for k, f in fams.items()})
+    print(json.dumps(out), flush=True)
 
 
 def main():
@@ -460,9 +601,17 @@ def main():
                     metavar=("FETCH_DIR", "WRITE_DIR", "CAL_FETCH_DIR", "CAL_WRITE_DIR", "BENCH_OUT"),
                     help="summarise rocprofv3 --pmc csv outputs into per-launch HBM bytes (JSON) "
                          "and exit; runs on the CPU")
+    ap.add_argument("--ensemble-summary", nargs=6,
+                    metavar=("TRACE_DIR", "FETCH_DIR", "WRITE_DIR", "CAL_FETCH_DIR", "CAL_WRITE_DIR",
+                             "BENCH_OUT"),
+                    help="summarise rocprofv3 kernel-trace and --pmc outputs of a --profile-form "
+                         "ensemble run, and exit")
     ap.add_argument("--trace-summary", nargs=2, metavar=("TRACE_DIR", "BENCH_OUT"),
                     help="summarise a rocprofv3 --kernel-trace csv output of this script (run "
                          "with the same --steps/--warmup) against its JSON line, and exit")
+    ap.add_argument("--profile-form", choices=PROFILE_FORMS,
+                    help="run one workload alone with the stored variants (for a profiler run) "
+                         "and print its per-kernel timings; no headline line")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain device copy of 1 GiB five times (the reference "
                          "kernel for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE) and exit")
@@ -473,6 +622,9 @@ def main():
         return
     if args.trace_summary:
         print(trace_summary(*args.trace_summary, args.steps, args.warmup))
+        return
+    if args.ensemble_summary:
+        print(ensemble_summary(*args.ensemble_summary))
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build runs a single problem (or ensemble) per GPU "
@@ -493,6 +645,9 @@ def main():
     import passivetracerflows as ptf
     dev = ptf.ROCm(0)
     autotune = False if args.no_autotune else ("retune" if args.retune else "cache")
+    if args.profile_form:
+        profile_form(dev, args.profile_form, args.steps, args.warmup, autotune)
+        return
 
     cfg, prob, secs, (m0, m1, v0, v1) = headline(dev, args.steps, args.warmup, autotune=autotune)
     s64 = prob.sol            # state after warmup + steps, for the mixed-form comparison

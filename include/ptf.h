@@ -221,8 +221,10 @@ int ptf_step_profiled(ptf_problem* prob, int64_t nsteps, double* slot_ms);
 
 /* ---- ensembles: many independent 2D problems, one graph-captured launch
  *      sequence per step (no reference counterpart: a batch of
- *      TracerAdvectionDiffusion.Problem(dev, TwoDAdvectingFlow(...)) with
- *      steady flows and RK4) ------------------------------------------------ */
+ *      TracerAdvectionDiffusion.Problem(dev, TwoDAdvectingFlow(...)), S:165-188,
+ *      each member with its own grid size, diffusivities and flow; RK4, ETDRK4 or
+ *      their filtered forms; steady flows, or per-step flows the caller refreshes
+ *      with ptf_ensemble_set_flow at the member's clock.t) ------------------------ */
 int ptf_ensemble_create(const ptf_desc* descs, int32_t count, ptf_ensemble** out);
 int ptf_ensemble_destroy(ptf_ensemble* ens);
 int ptf_ensemble_set_stream(ptf_ensemble* ens, void* hip_stream);
@@ -232,6 +234,12 @@ int ptf_ensemble_get_c(ptf_ensemble* ens, int32_t member, double* c);
 int ptf_ensemble_get_sol(ptf_ensemble* ens, int32_t member, double* sol);
 /* Advance every member nsteps steps; use_graph != 0 replays a captured hipGraph. */
 int ptf_ensemble_step(ptf_ensemble* ens, int64_t nsteps, int32_t use_graph);
+/* A member's clock (every member steps with the ensemble; t advances by that member's dt
+ * after each step, as [FF] stepforward! advances prob.clock), and a shared reset. */
+int ptf_ensemble_get_clock(ptf_ensemble* ens, int32_t member, double* t, int64_t* step);
+int ptf_ensemble_set_clock(ptf_ensemble* ens, double t, int64_t step);
+/* FP64 mass and variance of one member (as ptf_diagnostics for a problem). */
+int ptf_ensemble_diagnostics(ptf_ensemble* ens, int32_t member, double* mass, double* variance);
 int ptf_ensemble_autotune(ptf_ensemble* ens);
 int ptf_ensemble_num_slots(ptf_ensemble* ens, int32_t* nslots);
 int ptf_ensemble_slot_name(ptf_ensemble* ens, int32_t slot, char* buf, int32_t buflen);

@@ -108,13 +108,49 @@ def build(jobs: int | None = None, incremental: bool = False, verbose: bool = Tr
     return lib
 
 
+def build_asan(verbose: bool = True) -> str:
+    """Host-side AddressSanitizer build of the C-ABI runtime: csrc/ptf_api.hip compiled with
+    ASan on its host code only (the device code and the kernel registry objects are the
+    regular ones), linked into lib/libptf_hip_asan.so.  Needs a regular build's objects.
+    Load it with the clang ASan runtime preloaded (see tests/test_sanitizer.py); it is for
+    the CPU tests of the boundary (argument checks, error paths, host buffers)."""
+    units = _units()
+    regs = [o for o, _, _ in units if os.path.basename(o).startswith("reg_")]
+    missing = [o for o in regs if not os.path.exists(o)]
+    if missing:
+        raise RuntimeError("build the library first (missing " + os.path.basename(missing[0]) + ")")
+    obj = os.path.join(OBJDIR, "ptf_api_asan.o")
+    flags = [f for f in COMMON if f != "-O3"]
+    cmd = [HIPCC, *flags, "-O1", "-g", "-Xarch_host", "-fsanitize=address",
+           "-Xarch_host", "-fno-omit-frame-pointer", "-c", os.path.join(CSRC, "ptf_api.hip"), "-o", obj]
+    rc, err, dt = _run(cmd)
+    if rc != 0:
+        raise RuntimeError(f"hipcc (asan) failed:\n{err}")
+    lib = os.path.join(LIBDIR, "libptf_hip_asan.so")
+    clangxx = os.path.join(os.path.dirname(os.path.realpath(HIPCC)), "..", "lib", "llvm", "bin", "clang++")
+    if not os.path.exists(clangxx):
+        clangxx = "/opt/rocm/lib/llvm/bin/clang++"
+    link = [clangxx, "-shared", "-fPIC", "-fsanitize=address", "-shared-libasan", "-o", lib, obj, *regs,
+            "-L/opt/rocm/lib", "-lamdhip64", "-Wl,-rpath,/opt/rocm/lib"]
+    rc, err, _ = _run(link)
+    if rc != 0:
+        raise RuntimeError(f"asan link failed:\n{err}")
+    if verbose:
+        print(f"libptf_hip_asan: {lib} ({dt:.1f}s)", flush=True)
+    return lib
+
+
 if __name__ == "__main__":
     ap = argparse.ArgumentParser()
     ap.add_argument("--jobs", type=int, default=None)
     ap.add_argument("--incremental", action="store_true")
+    ap.add_argument("--asan", action="store_true",
+                    help="also link lib/libptf_hip_asan.so (host AddressSanitizer build of the runtime)")
     a = ap.parse_args()
     try:
         build(a.jobs, a.incremental)
+        if a.asan:
+            build_asan()
     except RuntimeError as e:
         print(e, file=sys.stderr)
         sys.exit(1)

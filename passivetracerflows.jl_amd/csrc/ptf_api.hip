@@ -128,11 +128,27 @@ static const char* kind_name(int kind) {
   return "?";
 }
 
-static const int kRadices[] = {16, 8, 4};  // tried in this order
+// elements per thread, tried in this order: the power-of-two layouts, then the one layout
+// each other length 2^a 3^b 5^c is compiled with (line_e)
+static const int kRadices[] = {16, 8, 4, 12, 20, 30, 6, 10, 15, 3, 5};
 static const int kColWidths[] = {16, 8, 32, 64, 4, 2, 1};
 static const int kNumColWidths = 7;
 static const int kRowWidths[] = {8, 16, 4, 2, 1, 1};
 static const int kNumRowWidths = 5;
+
+// compute units of the current device (persistent kernels size their grids by it)
+static int device_cus() {
+  static int cus = 0;
+  if (cus == 0) {
+    int dev = 0, n = 0;
+    if (hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && n > 0)
+      cus = n;
+    else
+      cus = 256;
+  }
+  return cus;
+}
 
 // ---------------------------------------------------------------------------
 // launch slots
@@ -157,7 +173,9 @@ struct Slot {
 
   bool is_col() const { return kind < 10; }
   bool has_occ() const { return kind == 3 || kind == 6 || kind == 13 || kind == 14; }
-  bool has_par() const { return kind == 13 || kind == 14; }
+  // par: the input-parallel advection form (kinds 13, 14) or the pipelined persistent
+  // stage-boundary kernel (kind 6)
+  bool has_par() const { return kind == 6 || kind == 13 || kind == 14; }
   int threads() const {
     if (fixed_threads > 0) return fixed_threads;
     if (is_col()) return W * (N / R);
@@ -165,6 +183,13 @@ struct Slot {
   }
   dim3 grid() const {
     if (fixed_threads > 0) return dim3(fixed_blocks, 1, 1);
+    if (is_col() && par) {
+      // persistent blocks: two waves per SIMD on every CU, at most one block per tile
+      const long long tiles = ((lines + W - 1) / W) * (long long)other * members;
+      const long long wpb = (threads() + 63) / 64;
+      const long long want = (long long)device_cus() * 8 / wpb;
+      return dim3((unsigned)std::max(1LL, std::min(tiles, want)), 1, 1);
+    }
     if (is_col()) return dim3((unsigned)((lines + W - 1) / W), (unsigned)other, (unsigned)members);
     return dim3((unsigned)((lines + W - 1) / W), (unsigned)members, 1);
   }
@@ -302,7 +327,6 @@ static int dalloc(DevBuf& b, size_t nbytes) {
   return PTF_OK;
 }
 
-static bool is_pow2(long long n) { return n > 0 && (n & (n - 1)) == 0; }
 
 // ---------------------------------------------------------------------------
 // group of B problems on one grid
@@ -364,6 +388,15 @@ static double kscale(int n, double L) {
   return fs / n;
 }
 
+// whether any kernel variant of `kind` was compiled for line length N (the registry
+// answers on the host, without a device)
+static bool len_compiled(bool dbl, int kind, int N) {
+  for (int R : kRadices)
+    for (int W : {1, 2, 4, 8, 16, 32, 64})
+      if (reg_lookup(dbl, kind, N, R, W)) return true;
+  return false;
+}
+
 static int validate_desc(const ptf_desc* d) {
   if (!d) return fail(PTF_ERR_INVALID, "desc is NULL");
   if (d->ndim < 1 || d->ndim > 3) return fail(PTF_ERR_INVALID, "ndim must be 1, 2 or 3");
@@ -388,13 +421,16 @@ static int validate_desc(const ptf_desc* d) {
     const long long n = d->n[a];
     if (n <= 0 || (n % 2)) return fail(PTF_ERR_INVALID, "grid sizes must be positive and even");
     if (!(d->L[a] > 0)) return fail(PTF_ERR_INVALID, "domain lengths must be positive");
-    if (!is_pow2(n))
-      return fail(PTF_ERR_UNSUPPORTED, "grid size " + std::to_string(n) +
-                                           " is not a power of two (supported: powers of two)");
+    const bool dbl = d->precision == PTF_FP64;
     if (a == 0 && (n < 16 || n > 4096))
       return fail(PTF_ERR_UNSUPPORTED, "nx must be in [16, 4096], got " + std::to_string(n));
     if (a > 0 && (n < 8 || n > 2048))
       return fail(PTF_ERR_UNSUPPORTED, "ny and nz must be in [8, 2048], got " + std::to_string(n));
+    if (!len_compiled(dbl, a == 0 ? K_ROW_C2R : K_COL_FWD, (int)n))
+      return fail(PTF_ERR_UNSUPPORTED,
+                  "grid size " + std::to_string(n) + " has no compiled transform (supported: "
+                  "powers of two, and 2^a m for m in {3, 5, 9, 25, 27, 125}; "
+                  "for nx the half length nx/2 must be one of these)");
   }
   return PTF_OK;
 }
@@ -430,6 +466,9 @@ static ColArgs<T> col_base(const Group& g, int axis) {
     a.lb = g.lb;
   }
   a.kscale_x = g.ks[0];
+  // blockIdx.y extent of this pass (see make_slot) and the member count
+  a.nother = g.ndim == 3 ? (axis == 1 ? (g.nz >> g.lb) : g.ny) : 1;
+  a.nmembers = g.B;
   return a;
 }
 
@@ -787,7 +826,7 @@ static int group_init(Group& g, const ptf_desc* descs, int count, bool allow_bri
   int lb = 0;
   if (allow_brick && d.ndim >= 2) {
     const long long nlast = d.ndim == 3 ? d.n[2] : d.n[1];
-    if (nlast >= 64) lb = 3;
+    if (nlast >= 64 && nlast % 8 == 0) lb = 3;   // whole bricks of 8 planes
   }
   int rc = group_init_once(g, descs, count, lb, plan_only);
   if (rc == PTF_ERR_UNSUPPORTED && lb > 0) rc = group_init_once(g, descs, count, 0, plan_only);
@@ -1245,6 +1284,10 @@ struct ptf_plan {
 struct ptf_ensemble {
   std::vector<std::unique_ptr<Group>> groups;
   std::vector<std::pair<int, int>> where;  // member -> (group, index in group)
+  // the members' clocks: every member steps with the ensemble; t advances by the member's
+  // dt after each step (t += dt, as FourierFlows' clock), kept per group (one dt per group)
+  std::vector<double> gt;
+  int64_t step = 0;
   hipStream_t stream = nullptr;
   bool own_stream = false;
   std::vector<hipStream_t> gstreams;
@@ -1805,6 +1848,8 @@ int ptf_get_L(ptf_problem* prob, double* L) {
 static int diagnostics_group(Group& g, int member, hipStream_t st, double* mass, double* var) {
   const size_t cs = 2 * g.esz;
   const char* base = (const char*)g.sol.p + (size_t)member * g.nS * cs;
+  // the mixed form's state is the compensated pair sol + sol_lo: both enter the FP64 sums
+  const char* lo = g.sol_lo.p ? (const char*)g.sol_lo.p + (size_t)member * g.nS * cs : nullptr;
   double s0re = 0, s0im = 0;
   if (g.dbl) {
     double v[2];
@@ -1813,21 +1858,24 @@ static int diagnostics_group(Group& g, int member, hipStream_t st, double* mass,
     s0re = v[0];
     s0im = v[1];
   } else {
-    float v[2];
+    float v[2], w[2] = {0.f, 0.f};
     HIPCHK(hipMemcpyAsync(v, base, 8, hipMemcpyDeviceToHost, st));
+    if (lo) HIPCHK(hipMemcpyAsync(w, lo, 8, hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
-    s0re = v[0];
-    s0im = v[1];
+    s0re = (double)v[0] + (double)w[0];
+    s0im = (double)v[1] + (double)w[1];
   }
   if (mass) *mass = s0re;
   if (var) {
     const int blocks = 1024;
     if (g.dbl)
       hipLaunchKernelGGL(variance_partial_kernel<double>, dim3(blocks), dim3(256), 0, st,
-                         (const cpx<double>*)base, g.nS, g.nxh, g.lb, (double*)g.red.p);
+                         (const cpx<double>*)base, (const cpx<double>*)nullptr, g.nS, g.nxh, g.lb,
+                         (double*)g.red.p);
     else
       hipLaunchKernelGGL(variance_partial_kernel<float>, dim3(blocks), dim3(256), 0, st,
-                         (const cpx<float>*)base, g.nS, g.nxh, g.lb, (double*)g.red.p);
+                         (const cpx<float>*)base, (const cpx<float>*)lo, g.nS, g.nxh, g.lb,
+                         (double*)g.red.p);
     HIPCHK(hipGetLastError());
     std::vector<double> part(2 * blocks);
     HIPCHK(hipMemcpyAsync(part.data(), g.red.p, part.size() * sizeof(double), hipMemcpyDeviceToHost, st));
@@ -1888,16 +1936,18 @@ int ptf_step_record(ptf_problem* prob, int64_t nsteps, int64_t every, int64_t ca
       double* dst = (double*)rec.p + 2 * r;
       if (g.dbl) {
         hipLaunchKernelGGL(variance_partial_kernel<double>, dim3(blocks), dim3(256), 0, st,
-                           (const cpx<double>*)g.sol.p, g.nS, g.nxh, g.lb, (double*)g.red.p);
+                           (const cpx<double>*)g.sol.p, (const cpx<double>*)nullptr, g.nS, g.nxh,
+                           g.lb, (double*)g.red.p);
         hipLaunchKernelGGL(diag_record_kernel<double>, dim3(1), dim3(64), 0, st,
-                           (const cpx<double>*)g.sol.p, (const double*)g.red.p, 2 * blocks,
-                           (double)g.nR, dst);
+                           (const cpx<double>*)g.sol.p, (const cpx<double>*)nullptr,
+                           (const double*)g.red.p, 2 * blocks, (double)g.nR, dst);
       } else {
         hipLaunchKernelGGL(variance_partial_kernel<float>, dim3(blocks), dim3(256), 0, st,
-                           (const cpx<float>*)g.sol.p, g.nS, g.nxh, g.lb, (double*)g.red.p);
+                           (const cpx<float>*)g.sol.p, (const cpx<float>*)g.sol_lo.p, g.nS, g.nxh,
+                           g.lb, (double*)g.red.p);
         hipLaunchKernelGGL(diag_record_kernel<float>, dim3(1), dim3(64), 0, st,
-                           (const cpx<float>*)g.sol.p, (const double*)g.red.p, 2 * blocks,
-                           (double)g.nR, dst);
+                           (const cpx<float>*)g.sol.p, (const cpx<float>*)g.sol_lo.p,
+                           (const double*)g.red.p, 2 * blocks, (double)g.nR, dst);
       }
       if (hipGetLastError() != hipSuccess) rc = fail(PTF_ERR_DEVICE, "diagnostics launch failed");
     }
@@ -2006,12 +2056,10 @@ int ptf_ensemble_create(const ptf_desc* descs, int32_t count, ptf_ensemble** out
     CHK(validate_desc(&descs[i]));
     if (descs[i].flow_kind == PTF_FLOW_SPECTRAL || descs[i].flow_kind == PTF_FLOW_ANALYTIC)
       return fail(PTF_ERR_UNSUPPORTED, "ensemble members take steady or per-step flows");
-    if (descs[i].stepper == PTF_FILTERED_RK4 || descs[i].stepper == PTF_FILTERED_ETDRK4)
-      return fail(PTF_ERR_UNSUPPORTED, "ensemble members take the RK4 or ETDRK4 stepper");
   }
   std::unique_ptr<ptf_ensemble> e(new ptf_ensemble());
   e->where.resize(count);
-  // group members with identical grid, precision, stepper, dt and nkappah
+  // group members with identical grid, precision, stepper (and filter), dt and nkappah
   std::vector<std::vector<int>> groups;
   for (int i = 0; i < count; ++i) {
     const ptf_desc& a = descs[i];
@@ -2019,7 +2067,9 @@ int ptf_ensemble_create(const ptf_desc* descs, int32_t count, ptf_ensemble** out
     for (size_t k = 0; k < groups.size(); ++k) {
       const ptf_desc& b = descs[groups[k][0]];
       bool same = a.ndim == b.ndim && a.precision == b.precision && a.stepper == b.stepper &&
-                  a.dt == b.dt && a.nkappah == b.nkappah;
+                  a.dt == b.dt && a.nkappah == b.nkappah && a.filter_order == b.filter_order &&
+                  a.filter_innerK == b.filter_innerK && a.filter_outerK == b.filter_outerK &&
+                  a.filter_tol == b.filter_tol;
       for (int x = 0; x < a.ndim && same; ++x) same = a.n[x] == b.n[x] && a.L[x] == b.L[x];
       if (same) {
         gi = (int)k;
@@ -2039,6 +2089,7 @@ int ptf_ensemble_create(const ptf_desc* descs, int32_t count, ptf_ensemble** out
     std::unique_ptr<Group> g(new Group());
     CHK(group_init(*g, ds.data(), (int)ds.size()));
     e->groups.push_back(std::move(g));
+    e->gt.push_back(0.0);
   }
   HIPCHK(hipStreamCreate(&e->stream));
   e->own_stream = true;
@@ -2150,9 +2201,48 @@ static int ensemble_record_step(ptf_ensemble* e) {
   return PTF_OK;
 }
 
+static void ensemble_advance_clock(ptf_ensemble* e, int64_t nsteps) {
+  for (size_t k = 0; k < e->groups.size(); ++k)
+    for (int64_t i = 0; i < nsteps; ++i) e->gt[k] += e->groups[k]->dt;
+  e->step += nsteps;
+}
+
+static int ensemble_step_impl(ptf_ensemble* ens, int64_t nsteps, int32_t use_graph);
+
 int ptf_ensemble_step(ptf_ensemble* ens, int64_t nsteps, int32_t use_graph) {
   if (!ens) return fail(PTF_ERR_INVALID, "ensemble is NULL");
   if (nsteps < 0) return fail(PTF_ERR_INVALID, "nsteps must be >= 0");
+  CHK(ensemble_step_impl(ens, nsteps, use_graph));
+  ensemble_advance_clock(ens, nsteps);
+  return PTF_OK;
+}
+
+int ptf_ensemble_get_clock(ptf_ensemble* ens, int32_t member, double* t, int64_t* step) {
+  Group* g;
+  int i;
+  CHK(member_of(ens, member, &g, &i));
+  if (!t || !step) return fail(PTF_ERR_INVALID, "NULL argument");
+  *t = ens->gt[ens->where[member].first];
+  *step = ens->step;
+  return PTF_OK;
+}
+
+int ptf_ensemble_set_clock(ptf_ensemble* ens, double t, int64_t step) {
+  if (!ens) return fail(PTF_ERR_INVALID, "ensemble is NULL");
+  for (double& x : ens->gt) x = t;
+  ens->step = step;
+  return PTF_OK;
+}
+
+int ptf_ensemble_diagnostics(ptf_ensemble* ens, int32_t member, double* mass, double* variance) {
+  Group* g;
+  int i;
+  CHK(member_of(ens, member, &g, &i));
+  if (!g->red.p) CHK(dalloc(g->red, 2 * 1024 * sizeof(double)));
+  return diagnostics_group(*g, i, ens->stream, mass, variance);
+}
+
+static int ensemble_step_impl(ptf_ensemble* ens, int64_t nsteps, int32_t use_graph) {
   for (auto& g : ens->groups) CHK(group_prime(*g, ens->stream));
   if (!use_graph) {
     for (int64_t i = 0; i < nsteps; ++i) CHK(ensemble_record_step(ens));

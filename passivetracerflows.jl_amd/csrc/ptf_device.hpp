@@ -14,8 +14,13 @@
 //   butterfly b reads in[b + r*N/Rp], multiplies element r by
 //   exp(DIR*2*pi*i*r*(b mod Ns)/(Ns*Rp)), does an Rp-point DFT and writes
 //   out[(b / Ns)*Ns*Rp + (b mod Ns) + r*Ns].
-// N = R0 * R^K with R0 <= R: the first pass (radix R0, Ns = 1, no twiddles)
-// works on registers, the K uniform passes follow.
+// A thread holds E = R elements; every radix of the line's plan divides E, and a
+// pass of radix Rp gives each thread the E/Rp butterflies b = j + s*P
+// (s < E/Rp), whose inputs j + (s + r*E/Rp)*P are exactly the thread's own
+// natural-layout registers.  So the first pass (Ns = 1, no twiddles) works on
+// registers, and the last pass (Ns*Rp = N) writes its outputs back to the
+// registers it read.  Powers of two use N = R0 * R^K (R0 <= R first); other
+// lengths 2^a 3^b 5^c use radices 2, 3, 4, 5, 8, 16 that divide E.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -140,24 +145,121 @@ __device__ __forceinline__ void dft16(cpx<T>* a) {
   for (int i = 0; i < 16; ++i) a[i] = o[i];
 }
 
+// Prime-length DFTs (3, 5) by the symmetric pairs sp_n = a_n + a_{Q-n},
+// dm_n = a_n - a_{Q-n}:
+//   X_k     = a_0 + sum_n cos(2 pi n k/Q) sp_n + DIR i sum_n sin(2 pi n k/Q) dm_n,
+//   X_{Q-k} = the same with the sine sum negated  (n, k = 1 .. (Q-1)/2).
+template <typename T>
+__device__ __forceinline__ cpx<T> axpy(T c, cpx<T> x, cpx<T> y) { return {y.re + c * x.re, y.im + c * x.im}; }
+
+template <int DIR, typename T>
+__device__ __forceinline__ void dft3(cpx<T>* a) {
+  constexpr double S1 = 0.8660254037844386467637;   // sin(2 pi/3)
+  const cpx<T> sp = a[1] + a[2], dm = a[1] - a[2];
+  const cpx<T> x0 = a[0] + sp;
+  const cpx<T> re = axpy((T)-0.5, sp, a[0]);
+  const cpx<T> t = mul_i<DIR>(cscale((T)S1, dm));
+  a[0] = x0;
+  a[1] = re + t;
+  a[2] = re - t;
+}
+
+template <int DIR, typename T>
+__device__ __forceinline__ void dft5(cpx<T>* a) {
+  constexpr double C1 = 0.3090169943749474241023, S1 = 0.9510565162951535721164;    // 2 pi/5
+  constexpr double C2 = -0.8090169943749474241023, S2 = 0.5877852522924731291687;   // 4 pi/5
+  const cpx<T> sp1 = a[1] + a[4], dm1 = a[1] - a[4];
+  const cpx<T> sp2 = a[2] + a[3], dm2 = a[2] - a[3];
+  const cpx<T> x0 = a[0] + sp1 + sp2;
+  const cpx<T> re1 = axpy((T)C2, sp2, axpy((T)C1, sp1, a[0]));
+  const cpx<T> im1 = axpy((T)S2, dm2, cscale((T)S1, dm1));
+  const cpx<T> re2 = axpy((T)C1, sp2, axpy((T)C2, sp1, a[0]));
+  const cpx<T> im2 = axpy((T)-S1, dm2, cscale((T)S2, dm1));
+  const cpx<T> t1 = mul_i<DIR>(im1), t2 = mul_i<DIR>(im2);
+  a[0] = x0;
+  a[1] = re1 + t1;
+  a[4] = re1 - t1;
+  a[2] = re2 + t2;
+  a[3] = re2 - t2;
+}
+
 template <int R, int DIR, typename T>
 __device__ __forceinline__ void dft(cpx<T>* a) {
   if constexpr (R == 2) dft2<DIR>(a[0], a[1]);
   else if constexpr (R == 4) dft4<DIR>(a[0], a[1], a[2], a[3]);
   else if constexpr (R == 8) dft8<DIR>(a);
   else if constexpr (R == 16) dft16<DIR>(a);
+  else if constexpr (R == 3) dft3<DIR>(a);
+  else if constexpr (R == 5) dft5<DIR>(a);
   else static_assert(R == 2, "unsupported radix");
 }
 
-constexpr int uniform_passes(int N, int R) {
-  int k = 0;
-  while (N > R && N % R == 0) {
-    N /= R;
-    ++k;
+// Pass radices of a line of N points held E per thread (each dividing E, product N),
+// smallest first; n = 0 when N cannot be split that way.  Greedy from the largest radix
+// that divides both E and what is left: for powers of two with E = R this is
+// R0 * R^K with the remainder R0 first.
+struct RadixPlan {
+  int n;
+  int r[24];
+};
+constexpr RadixPlan radix_plan(int N, int E) {
+  RadixPlan p{0, {}};
+  if (N < 2 || E < 2 || N % E != 0) return p;
+  const int pref[6] = {16, 8, 5, 4, 3, 2};
+  int tmp[24] = {};
+  int k = 0, rest = N;
+  while (rest > 1) {
+    int pick = 0;
+    for (int i = 0; i < 6; ++i) {
+      if (E % pref[i] == 0 && rest % pref[i] == 0) {
+        pick = pref[i];
+        break;
+      }
+    }
+    if (pick == 0 || k == 24) return RadixPlan{0, {}};
+    tmp[k++] = pick;
+    rest /= pick;
   }
-  return k;
+  for (int i = 0; i < k; ++i) p.r[i] = tmp[k - 1 - i];
+  p.n = k;
+  return p;
 }
-constexpr int ipow(int b, int e) { return e == 0 ? 1 : b * ipow(b, e - 1); }
+constexpr int radix_span(const RadixPlan& p, int pass) {   // product of the radices before `pass`
+  int s = 1;
+  for (int i = 0; i < pass; ++i) s *= p.r[i];
+  return s;
+}
+constexpr bool fft_ok(int N, int E) { return radix_plan(N, E).n > 0; }
+// workgroup barriers one line_fft<N, E> executes: one before the first pass's LDS writes,
+// two per middle pass, one before the last pass's LDS reads (none for a single pass)
+constexpr int fft_barriers(int N, int E) {
+  const int n = radix_plan(N, E).n;
+  return n >= 2 ? 2 * n - 2 : 0;
+}
+
+// Elements per thread of the one register layout compiled for a line length N that is
+// not a power of two: N = 2^a 3^b 5^c with E = 2^min(a,2) times the product of the
+// distinct odd primes (2^min(a,1) times it when both 3 and 5 divide N), at most 30;
+// 0 when N has another prime factor or is a power of two.
+constexpr int line_e(int N) {
+  if (N < 2) return 0;
+  int a = 0, n = N;
+  while (n % 2 == 0) {
+    n /= 2;
+    ++a;
+  }
+  int odd = 1;
+  const int primes[2] = {3, 5};
+  for (int i = 0; i < 2; ++i) {
+    if (n % primes[i] == 0) {
+      odd *= primes[i];
+      while (n % primes[i] == 0) n /= primes[i];
+    }
+  }
+  if (n != 1 || odd == 1) return 0;
+  const int e = (odd < 15 ? (1 << (a < 2 ? a : 2)) : (1 << (a < 1 ? a : 1))) * odd;
+  return e <= 30 && fft_ok(N, e) ? e : 0;
+}
 
 // LDS layouts: line c, element i
 template <int NL>
@@ -169,56 +271,79 @@ struct LayStrided {      // runtime strides: line-major (rows contiguous) or ele
   __device__ __forceinline__ int idx(int c, int i) const { return c * sc + i * si; }
 };
 
-// Line FFT of length N with radix R on P = N/R threads per line (see header).
-// tw[m] = exp(-2 pi i m / LT) with LT = N * twstride.  Starts with a barrier
-// before its first LDS write; ends after its last LDS read (callers that write
-// LDS afterwards must barrier first).  Every thread of the block must call it.
+// One pass of a line FFT (see the header): the E/RP butterflies of this thread on its
+// natural-layout registers a (already read from LDS unless this is the first pass),
+// outputs stored to LDS at their Stockham positions unless this is the last pass.
+template <typename T, int N, int E, int DIR, int PASS, class Lay>
+__device__ __forceinline__ void fft_pass(cpx<T> (&a)[E], cpx<T>* lds, const Lay& lay, int c, int j,
+                                         const cpx<T>* __restrict__ tw, int twstride) {
+  constexpr RadixPlan pl = radix_plan(N, E);
+  constexpr int P = N / E;
+  constexpr int RP = pl.r[PASS];
+  constexpr int B = E / RP;
+  constexpr int Ns = radix_span(pl, PASS);
+  constexpr bool kLast = PASS == pl.n - 1;
+  static_assert(E % RP == 0, "a radix must divide the elements per thread");
+  if constexpr (PASS > 0) {
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < E; ++q) a[q] = lds[lay.idx(c, j + q * P)];
+  }
+#pragma unroll
+  for (int s = 0; s < B; ++s) {
+    cpx<T> t[RP];
+#pragma unroll
+    for (int r = 0; r < RP; ++r) t[r] = a[s + r * B];
+    if constexpr (PASS > 0) {
+      const int m = (j + s * P) % Ns;
+      const int step = (N / (Ns * RP)) * twstride;
+#pragma unroll
+      for (int r = 1; r < RP; ++r) {
+        cpx<T> w = tw[r * m * step];
+        if constexpr (DIR > 0) w.im = -w.im;
+        t[r] = cmul(t[r], w);
+      }
+    }
+    dft<RP, DIR>(t);
+#pragma unroll
+    for (int r = 0; r < RP; ++r) a[s + r * B] = t[r];
+  }
+  if constexpr (!kLast) {
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < B; ++s) {
+      const int b = j + s * P;
+      const int base = (b / Ns) * Ns * RP + b % Ns;
+#pragma unroll
+      for (int r = 0; r < RP; ++r) lds[lay.idx(c, base + r * Ns)] = a[s + r * B];
+    }
+  }
+}
+
+template <typename T, int N, int E, int DIR, int PASS, class Lay>
+__device__ __forceinline__ void fft_passes(cpx<T> (&a)[E], cpx<T>* lds, const Lay& lay, int c, int j,
+                                           const cpx<T>* __restrict__ tw, int twstride) {
+  constexpr RadixPlan pl = radix_plan(N, E);
+  if constexpr (PASS < pl.n) {
+    fft_pass<T, N, E, DIR, PASS, Lay>(a, lds, lay, c, j, tw, twstride);
+    fft_passes<T, N, E, DIR, PASS + 1, Lay>(a, lds, lay, c, j, tw, twstride);
+  }
+}
+
+// Line FFT of length N with R elements per thread on P = N/R threads per line (see
+// header).  tw[m] = exp(-2 pi i m / LT) with LT = N * twstride.  With more than one
+// pass it starts with a barrier before its first LDS write and ends after its last LDS
+// read (callers that write LDS afterwards must barrier first); every thread of the block
+// must call it.
 template <typename T, int N, int R, int DIR, class Lay>
 __device__ __forceinline__ void line_fft(cpx<T> (&a)[R], cpx<T>* lds, const Lay& lay, int c, int j,
                                          const cpx<T>* __restrict__ tw, int twstride) {
-  constexpr int P = N / R;
-  constexpr int K = uniform_passes(N, R);
-  constexpr int R0 = N / ipow(R, K);
-  static_assert(R0 * ipow(R, K) == N && R0 >= 2 && R0 <= R, "N must be R0 * R^K");
-  if constexpr (K == 0) {
+  constexpr RadixPlan pl = radix_plan(N, R);
+  static_assert(pl.n > 0, "N must split into radices that divide R");
+  if constexpr (pl.n == 1) {
     dft<R, DIR>(a);
-    return;
   } else {
-    constexpr int B = R / R0;
-    __syncthreads();
-#pragma unroll
-    for (int b = 0; b < B; ++b) {
-      cpx<T> t[R0];
-#pragma unroll
-      for (int r = 0; r < R0; ++r) t[r] = a[b + r * B];
-      dft<R0, DIR>(t);
-      const int jj = j + b * P;
-#pragma unroll
-      for (int r = 0; r < R0; ++r) lds[lay.idx(c, jj * R0 + r)] = t[r];
-    }
-    int Ns = R0;
-#pragma unroll
-    for (int p = 1; p <= K; ++p) {
-      __syncthreads();
-#pragma unroll
-      for (int r = 0; r < R; ++r) a[r] = lds[lay.idx(c, j + r * P)];
-      const int m = j % Ns;
-      const int step = (N / (Ns * R)) * twstride;
-#pragma unroll
-      for (int r = 1; r < R; ++r) {
-        cpx<T> w = tw[r * m * step];
-        if constexpr (DIR > 0) w.im = -w.im;
-        a[r] = cmul(a[r], w);
-      }
-      dft<R, DIR>(a);
-      if (p < K) {
-        __syncthreads();
-        const int base = (j / Ns) * Ns * R + m;
-#pragma unroll
-        for (int r = 0; r < R; ++r) lds[lay.idx(c, base + r * Ns)] = a[r];
-        Ns *= R;
-      }
-    }
+    fft_passes<T, N, R, DIR, 0, Lay>(a, lds, lay, c, j, tw, twstride);
   }
 }
 

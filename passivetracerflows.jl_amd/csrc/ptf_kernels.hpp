@@ -53,6 +53,10 @@ struct ColArgs {
   //            column = kx*2^lbc + (i_last mod 2^lbc), and blockIdx.y is i_last >> lbc
   int lb;
   int lbc;
+  // line sets along blockIdx.y and members (blockIdx.z): the tile space of the persistent
+  // pipelined kernel, which is launched on a 1D grid
+  int nother;
+  int nmembers;
 };
 
 // Occupancy range for a block of `threads` threads using `lds` bytes of LDS:
@@ -368,6 +372,217 @@ void col_kernel(ColArgs<T> a) {
 }
 
 // --------------------------------------------------------------------------
+// software-pipelined stage-boundary kernel (COL_STAGE_D1 work, persistent blocks)
+// --------------------------------------------------------------------------
+// The same work as col_kernel<..., COL_STAGE_D1, ...>: forward FFT of in0 along the line,
+// the RK4/ETDRK4 stage update, then IFFT and IFFT(i k .) of the next stage input.  A
+// block stays resident and walks over tiles (TX columns of one line set each); while it
+// transforms tile i it already has the loads of tile i+1 in flight: the line and, for
+// RK4, the stage operands, held in registers.  So a block's memory latency overlaps its
+// own compute instead of needing other blocks to fill the gap, which the register
+// budget of FP64 lines caps at a few waves per SIMD.
+//
+// Tiles are numbered x fastest (column tile), then blockIdx.y's dimension (`other`), then
+// members; XCD k takes a contiguous range of tiles (blocks are dispatched round-robin over
+// the 8 XCDs) and its blocks interleave over that range, so tiles that share cache lines
+// run at the same time on one L2.
+template <int N, int R, int TX>
+constexpr int pipe_threads() { return TX * (N / R); }
+
+template <typename T, int N, int R, int TX, int OCC = 0>
+__global__ __attribute__((amdgpu_flat_work_group_size(1, TX * (N / R)),
+                          amdgpu_waves_per_eu(1, OCC > 0 ? OCC : 2)))
+void col_pipe_kernel(ColArgs<T> a) {
+  constexpr int P = N / R;
+  const int tiles_x = (a.ncols + TX - 1) / TX;
+  const int tiles_y = a.nother;
+  const int tiles_total = tiles_x * tiles_y * a.nmembers;
+  using Lay = LayInterleaved<TX>;
+  const Lay lay{};
+  __shared__ cpx<T> lds[(N > R) ? N * TX : 1];
+  __shared__ cpx<T> twl[col_tw_lds<N, R>() ? N : 1];
+  const int t = threadIdx.x;
+  if constexpr (col_tw_lds<N, R>()) {
+    for (int i = t; i < N; i += TX * P) twl[i] = a.tw[i * a.twstride];
+  }
+  const int lb = a.lb;
+  const int rb = t & ((1 << lb) - 1);
+  const int rest = t >> lb;
+  const int c = rest % TX;
+  const int jh = rest / TX;
+  const int j = rb + (jh << lb);
+  auto fft = [&](auto dir_c, cpx<T>(&x)[R]) {
+    constexpr int D = decltype(dir_c)::value;
+    if constexpr (col_tw_lds<N, R>()) line_fft<T, N, R, D, Lay>(x, lds, lay, c, j, twl, 1);
+    else line_fft<T, N, R, D, Lay>(x, lds, lay, c, j, a.tw, a.twstride);
+  };
+  const uint32_t s = (uint32_t)a.stride;
+
+  // this block's tiles: XCD range [lo, hi), visited with stride = blocks on the XCD (with
+  // fewer than 8 blocks, each block is its own range)
+  const unsigned nb = gridDim.x;
+  const unsigned kXcd = nb < 8u ? nb : 8u;
+  const unsigned xcd = blockIdx.x % kXcd, local = blockIdx.x / kXcd;
+  const unsigned per_xcd_blocks = nb / kXcd + (xcd < nb % kXcd ? 1u : 0u);
+  const unsigned per = (unsigned)tiles_total / kXcd, remt = (unsigned)tiles_total % kXcd;
+  const unsigned lo = xcd * per + (xcd < remt ? xcd : remt);
+  const unsigned hi = lo + per + (xcd < remt ? 1u : 0u);
+
+  struct Tile {
+    int bx, by, member, kx, kxc;
+    bool valid;
+    long long ubase;
+  };
+  auto tile_at = [&](unsigned L) {
+    Tile u;
+    u.bx = (int)(L % (unsigned)tiles_x);
+    u.by = (int)((L / (unsigned)tiles_x) % (unsigned)tiles_y);
+    u.member = (int)(L / ((unsigned)tiles_x * (unsigned)tiles_y));
+    u.kx = u.bx * TX + c;
+    u.valid = u.kx < a.ncols;
+    u.kxc = u.valid ? u.kx : a.ncols - 1;
+    u.ubase = (long long)u.member * a.mstride + (long long)u.by * a.ostride;
+    return u;
+  };
+  auto off = [&](const Tile& u, int q) -> uint32_t {
+    return ((uint32_t)u.kxc << lb) + (uint32_t)rb + (uint32_t)(jh + q * (P >> lb)) * s;
+  };
+  auto load_line = [&](const Tile& u, cpx<T>(&v)[R]) {
+    const cpx<T>* in0 = a.in0 + u.ubase;
+#pragma unroll
+    for (int q = 0; q < R; ++q) v[q] = in0[off(u, q)];
+  };
+
+  const bool rk4 = a.st.kind == EPI_RK4;
+  auto stage_dispatch = [&](auto body) {
+    switch (a.st.stage) {
+      case 0: body(std::integral_constant<int, 0>{}); break;
+      case 1: body(std::integral_constant<int, 1>{}); break;
+      case 2: body(std::integral_constant<int, 2>{}); break;
+      default: body(std::integral_constant<int, 3>{}); break;
+    }
+  };
+
+  // the transform of the next stage input v (already updated) and its stores
+  auto next_stage_inverse = [&](const Tile& u, cpx<T>(&v)[R]) {
+    cpx<T> keep[R];
+#pragma unroll
+    for (int q = 0; q < R; ++q) keep[q] = v[q];
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+      const T k = (T)(fftfreq_index(j + q * P, N) * a.kscale);
+      v[q] = times_ik(v[q], k);
+    }
+    fft(std::integral_constant<int, 1>{}, v);
+    if (u.valid) {
+      cpx<T>* o1 = a.out1 + u.ubase;
+#pragma unroll
+      for (int q = 0; q < R; ++q) o1[off(u, q)] = v[q];
+    }
+#pragma unroll
+    for (int q = 0; q < R; ++q) v[q] = keep[q];
+    fft(std::integral_constant<int, 1>{}, v);
+    if (u.valid) {
+      cpx<T>* o0 = a.out0 + u.ubase;
+#pragma unroll
+      for (int q = 0; q < R; ++q) o0[off(u, q)] = v[q];
+    }
+  };
+
+  unsigned L = lo + local;
+  if (L >= hi) return;
+  if (rk4) {
+    stage_dispatch([&](auto stc) {
+      constexpr int ST = decltype(stc)::value;
+      Tile cur = tile_at(L);
+      cpx<T> v[R];
+      Rk4Ops<T> ops[R];
+      load_line(cur, v);
+#pragma unroll
+      for (int q = 0; q < R; ++q) ops[q] = rk4_load<T, ST>(a.st, cur.ubase, off(cur, q));
+      for (;;) {
+        const unsigned Ln = L + per_xcd_blocks;
+        const bool more = Ln < hi;
+        Tile nxt = cur;
+        cpx<T> vn[R];
+        Rk4Ops<T> opsn[R];
+        if (more) {
+          // the next tile's loads go out before this tile's work and stores
+          nxt = tile_at(Ln);
+          load_line(nxt, vn);
+#pragma unroll
+          for (int q = 0; q < R; ++q) opsn[q] = rk4_load<T, ST>(a.st, nxt.ubase, off(nxt, q));
+        }
+        fft(std::integral_constant<int, -1>{}, v);
+        const OpParams opm = a.st.op[cur.member];
+        const int ikx = cur.kx >> a.lbc;
+        const int iklast = (cur.by << a.lbc) + (cur.kx & ((1 << a.lbc) - 1));
+#pragma unroll
+        for (int q = 0; q < R; ++q) {
+          const int ia = j + q * P;
+          const int iky = a.axis == 2 ? cur.by : ia;
+          const int ikz = a.axis == 2 ? ia : iklast;
+          v[q] = rk4_finish<T, ST>(a.st, cur.ubase, off(cur, q), opm, ikx, iky, ikz, v[q], ops[q],
+                                   cur.valid);
+        }
+        next_stage_inverse(cur, v);
+        if (!more) break;
+        L = Ln;
+        cur = nxt;
+#pragma unroll
+        for (int q = 0; q < R; ++q) {
+          v[q] = vn[q];
+          ops[q] = opsn[q];
+        }
+      }
+    });
+  } else {
+    // ETDRK4: up to 9 operands per element, loaded after the transform; the next line is
+    // still prefetched
+    stage_dispatch([&](auto stc) {
+      constexpr int ST = decltype(stc)::value;
+      Tile cur = tile_at(L);
+      cpx<T> v[R];
+      load_line(cur, v);
+      for (;;) {
+        const unsigned Ln = L + per_xcd_blocks;
+        const bool more = Ln < hi;
+        Tile nxt = cur;
+        cpx<T> vn[R];
+        if (more) {
+          nxt = tile_at(Ln);
+          load_line(nxt, vn);
+        }
+        fft(std::integral_constant<int, -1>{}, v);
+        const int ikx = cur.kx >> a.lbc;
+        const int iklast = (cur.by << a.lbc) + (cur.kx & ((1 << a.lbc) - 1));
+        constexpr int kChunk = R < 4 ? R : 4;
+#pragma unroll
+        for (int q0 = 0; q0 < R; q0 += kChunk) {
+          EtdOps<T> o[kChunk];
+#pragma unroll
+          for (int i = 0; i < kChunk; ++i) o[i] = etd_load<T, ST>(a.st, cur.ubase, off(cur, q0 + i));
+#pragma unroll
+          for (int i = 0; i < kChunk; ++i) {
+            const int q = q0 + i;
+            const int ia = j + q * P;
+            const int iky = a.axis == 2 ? cur.by : ia;
+            const int ikz = a.axis == 2 ? ia : iklast;
+            v[q] = etd_finish<T, ST>(a.st, cur.ubase, off(cur, q), ikx, iky, ikz, v[q], o[i], cur.valid);
+          }
+        }
+        next_stage_inverse(cur, v);
+        if (!more) break;
+        L = Ln;
+        cur = nxt;
+#pragma unroll
+        for (int q = 0; q < R; ++q) v[q] = vn[q];
+      }
+    });
+  }
+}
+
+// --------------------------------------------------------------------------
 // row (x) kernels
 // --------------------------------------------------------------------------
 enum {
@@ -662,12 +877,12 @@ void row_kernel(RowArgs<T> a) {
 // transformed side by side.  The block is NIN groups of NL * M/R threads (whole
 // wavefronts); group d loads its spectral rows and flow values at once and
 // forms the product flow_d * C2R(D_d in_d) for the block's NL rows.  Groups
-// 1.. hand their products over through LDS and end; group 0 forms
-// ((-u cx) - v cy) - w cz and performs the R2C.  A workgroup barrier waits only
-// on the wavefronts that are still running, so group 0 keeps using barriers
-// after the others have ended.  Compared with row_kernel this trades idle lanes
-// in the R2C for fewer registers per lane and all loads of a block in flight
-// together.
+// 1.. hand their products over through LDS; group 0 forms ((-u cx) - v cy) - w cz
+// and performs the R2C, while the other groups only take part in its workgroup
+// barriers (the same number, counted from the R2C's radix plan), so every wavefront
+// of the block reaches every barrier and none ends early.  Compared with row_kernel
+// this trades idle lanes in the R2C for fewer registers per lane and all loads of a
+// block in flight together.
 template <typename T, int NX, int R, int NL, int NIN>
 constexpr int adv_par_threads() { return NIN * NL * ((NX / 2) / R); }
 template <typename T, int NX, int R, int NL, int NIN>
@@ -753,7 +968,12 @@ void row_adv_par_kernel(RowArgs<T> a) {
     for (int q = 0; q < R; ++q) lds[lay.idx(c, j + q * P)] = f[q];
   }
   __syncthreads();
-  if (grp != 0) return;
+  if (grp != 0) {
+    // the barriers of group 0's forward transform and of its Z[M-k] exchange below
+    constexpr int kBarriers = fft_barriers(M, R) + 2;
+    for (int i = 0; i < kBarriers; ++i) __syncthreads();
+    return;
+  }
 #pragma unroll
   for (int q = 0; q < R; ++q) {
     const int i = lay.idx(c, j + q * P);
@@ -766,7 +986,7 @@ void row_adv_par_kernel(RowArgs<T> a) {
     }
     f[q] = v;
   }
-  // forward half-length FFT of f[2m] + i f[2m+1] (barriers now among group 0 only)
+  // forward half-length FFT of f[2m] + i f[2m+1] (the other groups match its barriers)
   if constexpr (row_tw_lds<NX>()) line_fft<T, M, R, -1, Lay>(f, lds, lay, c, j, twl, 2);
   else line_fft<T, M, R, -1, Lay>(f, lds, lay, c, j, a.tw, 2 * a.twstride);
   __syncthreads();

@@ -48,10 +48,25 @@ constexpr int row_nin(int k) { return k == K_ROW_ADV2 ? 2 : k == K_ROW_ADV3 ? 3 
 // kinds that are also built with a 3-waves/SIMD register budget (OCC = 3)
 constexpr bool has_occ(int k) { return k == 3 || k == 6 || k == 13 || k == 14; }
 
+// Lengths that are not powers of two are compiled with one register layout (R =
+// line_e(N) elements per thread), the default register budget, and one width: the
+// largest power of two (at most wmax) that keeps a block within 128 threads.
+constexpr bool is_pow2_len(int n) { return n > 0 && (n & (n - 1)) == 0; }
+constexpr int reduced_width(int P, int wmax) {
+  int w = 1;
+  while (w * 2 <= wmax && w * 2 * P <= 128) w *= 2;
+  return w;
+}
+constexpr bool reduced_ok(int N, int R, int width, int wmax, int P, int OCC) {
+  return R == line_e(N) && OCC == 0 && width == reduced_width(P, wmax);
+}
+
 template <typename T, int N, int R, int TX, int KIND, int OCC>
 const void* colp() {
   constexpr int P = N / R;
-  constexpr bool ok = R <= N && (R >= 4) && TX * P >= 16 && TX * P <= 1024 &&
+  constexpr bool ok = R <= N && (R >= 4 || !is_pow2_len(N)) && N % R == 0 && fft_ok(N, R) &&
+                      TX * P >= 16 && TX * P <= 1024 &&
+                      (is_pow2_len(N) || reduced_ok(N, R, TX, 64, P, OCC)) &&
                       (long long)col_lds_elems<T, N, R, TX>() * (long long)sizeof(cpx<T>) <= 163840;
   if constexpr (ok) {
     return reinterpret_cast<const void*>(&col_kernel<T, N, R, TX, col_mode(KIND), col_dir(KIND), OCC>);
@@ -64,7 +79,9 @@ template <typename T, int NX, int R, int NL, int KIND, int OCC>
 const void* rowp() {
   constexpr int M = NX / 2;
   constexpr int P = M / R;
-  constexpr bool ok = R <= M && (R >= 4) && NL * P >= 16 && NL * P <= 1024 &&
+  constexpr bool ok = R <= M && (R >= 4 || !is_pow2_len(M)) && M % R == 0 && fft_ok(M, R) &&
+                      NL * P >= 16 && NL * P <= 1024 &&
+                      (is_pow2_len(M) || reduced_ok(M, R, NL, 16, P, OCC)) &&
                       (long long)NL * (M + 1) * (long long)sizeof(cpx<T>) + row_tw_bytes<T, NX>() <= 163840;
   if constexpr (ok) {
     return reinterpret_cast<const void*>(&row_kernel<T, NX, R, NL, row_mode(KIND), row_nin(KIND), OCC>);
@@ -80,7 +97,8 @@ const void* rowpar() {
     constexpr int NIN = row_nin(KIND);
     constexpr int M = NX / 2;
     constexpr int P = M / R;
-    constexpr bool ok = R <= M && R >= 4 && (NL * P) % 64 == 0 && NIN * NL * P <= 1024 &&
+    constexpr bool ok = is_pow2_len(M) && R <= M && R >= 4 && (NL * P) % 64 == 0 &&
+                        NIN * NL * P <= 1024 &&
                         adv_par_lds<T, NX, R, NL, NIN>() <= 163840;
     if constexpr (ok) {
       return reinterpret_cast<const void*>(&row_adv_par_kernel<T, NX, R, NL, NIN, OCC>);
@@ -92,9 +110,32 @@ const void* rowpar() {
   }
 }
 
+// pipelined persistent stage-boundary kernel (col_pipe_kernel), kind 6 with par = 1:
+// power-of-two lines of 64..1024 points, radix 4 or 8, up to 4 columns per block, the
+// default register budget (it holds two tiles' data)
+template <typename T, int N, int R, int TX, int OCC>
+const void* pipep() {
+  constexpr int P = N / R;
+  constexpr bool ok = OCC == 0 && is_pow2_len(N) && N >= 64 && N <= 1024 && (R == 4 || R == 8) && TX <= 4 &&
+                      TX * P >= 32 && TX * P <= 512 &&
+                      (long long)col_lds_elems<T, N, R, TX>() * (long long)sizeof(cpx<T>) <= 65536;
+  if constexpr (ok) {
+    return reinterpret_cast<const void*>(&col_pipe_kernel<T, N, R, TX, OCC>);
+  } else {
+    return nullptr;
+  }
+}
+
 template <typename T, int N, int R, int KIND, int OCC>
 const void* by_w_par(int W) {
-  if constexpr (KIND >= 10) {
+  if constexpr (KIND == K_COL_STAGE_D1) {
+    switch (W) {
+      case 1: return pipep<T, N, R, 1, OCC>();
+      case 2: return pipep<T, N, R, 2, OCC>();
+      case 4: return pipep<T, N, R, 4, OCC>();
+      default: return nullptr;
+    }
+  } else if constexpr (KIND >= 10) {
     switch (W) {
       case 1: return rowpar<T, N, R, 1, KIND, OCC>();
       case 2: return rowpar<T, N, R, 2, KIND, OCC>();
@@ -156,11 +197,19 @@ const void* by_w(int W) {
 
 template <typename T, int N, int KIND>
 const void* by_r(int R, int W) {
-  switch (R) {
-    case 4: return by_w<T, N, 4, KIND>(W);
-    case 8: return by_w<T, N, 8, KIND>(W);
-    case 16: return by_w<T, N, 16, KIND>(W);
-    default: return nullptr;
+  constexpr int L = KIND < 10 ? N : N / 2;   // the transformed length (rows: nx/2)
+  if constexpr (is_pow2_len(L)) {
+    switch (R) {
+      case 4: return by_w<T, N, 4, KIND>(W);
+      case 8: return by_w<T, N, 8, KIND>(W);
+      case 16: return by_w<T, N, 16, KIND>(W);
+      default: return nullptr;
+    }
+  } else if constexpr (line_e(L) > 0) {
+    if (R == line_e(L)) return by_w<T, N, line_e(L), KIND>(W);
+    return nullptr;
+  } else {
+    return nullptr;
   }
 }
 
@@ -184,6 +233,45 @@ const void* by_n(int N, int R, int W) {
       case 512: return in_part<T, 512, KIND, 6>(R, W);
       case 1024: return in_part<T, 1024, KIND, 7>(R, W);
       case 2048: return in_part<T, 2048, KIND, 8>(R, W);
+      // lengths 2^a m, m in {3, 5, 9, 25, 27, 125} (one register layout each)
+      case 12: return in_part<T, 12, KIND, 9>(R, W);
+      case 18: return in_part<T, 18, KIND, 10>(R, W);
+      case 20: return in_part<T, 20, KIND, 11>(R, W);
+      case 24: return in_part<T, 24, KIND, 12>(R, W);
+      case 36: return in_part<T, 36, KIND, 13>(R, W);
+      case 40: return in_part<T, 40, KIND, 14>(R, W);
+      case 48: return in_part<T, 48, KIND, 15>(R, W);
+      case 50: return in_part<T, 50, KIND, 16>(R, W);
+      case 54: return in_part<T, 54, KIND, 17>(R, W);
+      case 72: return in_part<T, 72, KIND, 18>(R, W);
+      case 80: return in_part<T, 80, KIND, 19>(R, W);
+      case 96: return in_part<T, 96, KIND, 20>(R, W);
+      case 100: return in_part<T, 100, KIND, 21>(R, W);
+      case 108: return in_part<T, 108, KIND, 22>(R, W);
+      case 144: return in_part<T, 144, KIND, 23>(R, W);
+      case 160: return in_part<T, 160, KIND, 24>(R, W);
+      case 192: return in_part<T, 192, KIND, 25>(R, W);
+      case 200: return in_part<T, 200, KIND, 26>(R, W);
+      case 216: return in_part<T, 216, KIND, 27>(R, W);
+      case 250: return in_part<T, 250, KIND, 28>(R, W);
+      case 288: return in_part<T, 288, KIND, 29>(R, W);
+      case 320: return in_part<T, 320, KIND, 30>(R, W);
+      case 384: return in_part<T, 384, KIND, 31>(R, W);
+      case 400: return in_part<T, 400, KIND, 32>(R, W);
+      case 432: return in_part<T, 432, KIND, 33>(R, W);
+      case 500: return in_part<T, 500, KIND, 34>(R, W);
+      case 576: return in_part<T, 576, KIND, 35>(R, W);
+      case 640: return in_part<T, 640, KIND, 36>(R, W);
+      case 768: return in_part<T, 768, KIND, 37>(R, W);
+      case 800: return in_part<T, 800, KIND, 38>(R, W);
+      case 864: return in_part<T, 864, KIND, 39>(R, W);
+      case 1000: return in_part<T, 1000, KIND, 40>(R, W);
+      case 1152: return in_part<T, 1152, KIND, 41>(R, W);
+      case 1280: return in_part<T, 1280, KIND, 42>(R, W);
+      case 1536: return in_part<T, 1536, KIND, 43>(R, W);
+      case 1600: return in_part<T, 1600, KIND, 44>(R, W);
+      case 1728: return in_part<T, 1728, KIND, 45>(R, W);
+      case 2000: return in_part<T, 2000, KIND, 46>(R, W);
       default: return nullptr;
     }
   } else {
@@ -197,6 +285,48 @@ const void* by_n(int N, int R, int W) {
       case 1024: return in_part<T, 1024, KIND, 6>(R, W);
       case 2048: return in_part<T, 2048, KIND, 7>(R, W);
       case 4096: return in_part<T, 4096, KIND, 8>(R, W);
+      // nx = 2 M with M = 2^a m, m in {3, 5, 9, 25, 27, 125} (one register layout each)
+      case 24: return in_part<T, 24, KIND, 9>(R, W);
+      case 36: return in_part<T, 36, KIND, 10>(R, W);
+      case 40: return in_part<T, 40, KIND, 11>(R, W);
+      case 48: return in_part<T, 48, KIND, 12>(R, W);
+      case 50: return in_part<T, 50, KIND, 13>(R, W);
+      case 54: return in_part<T, 54, KIND, 14>(R, W);
+      case 72: return in_part<T, 72, KIND, 15>(R, W);
+      case 80: return in_part<T, 80, KIND, 16>(R, W);
+      case 96: return in_part<T, 96, KIND, 17>(R, W);
+      case 100: return in_part<T, 100, KIND, 18>(R, W);
+      case 108: return in_part<T, 108, KIND, 19>(R, W);
+      case 144: return in_part<T, 144, KIND, 20>(R, W);
+      case 160: return in_part<T, 160, KIND, 21>(R, W);
+      case 192: return in_part<T, 192, KIND, 22>(R, W);
+      case 200: return in_part<T, 200, KIND, 23>(R, W);
+      case 216: return in_part<T, 216, KIND, 24>(R, W);
+      case 250: return in_part<T, 250, KIND, 25>(R, W);
+      case 288: return in_part<T, 288, KIND, 26>(R, W);
+      case 320: return in_part<T, 320, KIND, 27>(R, W);
+      case 384: return in_part<T, 384, KIND, 28>(R, W);
+      case 400: return in_part<T, 400, KIND, 29>(R, W);
+      case 432: return in_part<T, 432, KIND, 30>(R, W);
+      case 500: return in_part<T, 500, KIND, 31>(R, W);
+      case 576: return in_part<T, 576, KIND, 32>(R, W);
+      case 640: return in_part<T, 640, KIND, 33>(R, W);
+      case 768: return in_part<T, 768, KIND, 34>(R, W);
+      case 800: return in_part<T, 800, KIND, 35>(R, W);
+      case 864: return in_part<T, 864, KIND, 36>(R, W);
+      case 1000: return in_part<T, 1000, KIND, 37>(R, W);
+      case 1152: return in_part<T, 1152, KIND, 38>(R, W);
+      case 1280: return in_part<T, 1280, KIND, 39>(R, W);
+      case 1536: return in_part<T, 1536, KIND, 40>(R, W);
+      case 1600: return in_part<T, 1600, KIND, 41>(R, W);
+      case 1728: return in_part<T, 1728, KIND, 42>(R, W);
+      case 2000: return in_part<T, 2000, KIND, 43>(R, W);
+      case 2304: return in_part<T, 2304, KIND, 44>(R, W);
+      case 2560: return in_part<T, 2560, KIND, 45>(R, W);
+      case 3072: return in_part<T, 3072, KIND, 46>(R, W);
+      case 3200: return in_part<T, 3200, KIND, 47>(R, W);
+      case 3456: return in_part<T, 3456, KIND, 48>(R, W);
+      case 4000: return in_part<T, 4000, KIND, 49>(R, W);
       default: return nullptr;
     }
   }

@@ -87,8 +87,8 @@ __global__ void copy_kernel(T* __restrict__ out, const T* __restrict__ in, long 
 // 1 at kx = 0 and nx/2, 2 elsewhere), one (sum, compensation) pair per block;
 // the field may be in the brick layout (kx of element i is (i >> lb) mod nxh).
 template <typename T>
-__global__ void variance_partial_kernel(const cpx<T>* __restrict__ sol, long long n, int nxh,
-                                        int lb, double* __restrict__ partial) {
+__global__ void variance_partial_kernel(const cpx<T>* __restrict__ sol, const cpx<T>* __restrict__ sol_lo,
+                                        long long n, int nxh, int lb, double* __restrict__ partial) {
   __shared__ double ssum[256];
   __shared__ double scomp[256];
   double sum = 0.0, comp = 0.0;
@@ -96,8 +96,14 @@ __global__ void variance_partial_kernel(const cpx<T>* __restrict__ sol, long lon
        i += (long long)gridDim.x * blockDim.x) {
     const int kx = (int)((i >> lb) % nxh);   // brick layout: 2^lb planes innermost
     const double w = (kx == 0 || kx == nxh - 1) ? 1.0 : 2.0;
+    // the mixed form's compensated state is sol + sol_lo, summed in FP64
     const cpx<T> v = sol[i];
-    const double x = w * ((double)v.re * (double)v.re + (double)v.im * (double)v.im);
+    double vr = (double)v.re, vi = (double)v.im;
+    if (sol_lo) {
+      vr += (double)sol_lo[i].re;
+      vi += (double)sol_lo[i].im;
+    }
+    const double x = w * (vr * vr + vi * vi);
     const double tt = sum + x;
     if (fabs(sum) >= fabs(x)) comp += (sum - tt) + x;
     else comp += (x - tt) + sum;
@@ -246,12 +252,13 @@ __global__ void dealias_kernel(cpx<T>* __restrict__ f, long long n, int nxh, int
 }
 
 // Mass and variance record from the partial sums of variance_partial_kernel:
-// rec[0] = Re sol[0] (mass = sum of c), rec[1] = variance.  One thread sums the
+// rec[0] = Re sol[0] (mass = sum of c; sol + sol_lo in the mixed form), rec[1] = variance.  One thread sums the
 // partials in order with Neumaier compensation, as the host does for a single
 // diagnostics call, so both give the same value.
 template <typename T>
-__global__ void diag_record_kernel(const cpx<T>* __restrict__ sol, const double* __restrict__ partial,
-                                   int npartial, double npoints, double* __restrict__ rec) {
+__global__ void diag_record_kernel(const cpx<T>* __restrict__ sol, const cpx<T>* __restrict__ sol_lo,
+                                   const double* __restrict__ partial, int npartial, double npoints,
+                                   double* __restrict__ rec) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   double sum = 0.0, comp = 0.0;
   for (int i = 0; i < npartial; ++i) {
@@ -260,7 +267,11 @@ __global__ void diag_record_kernel(const cpx<T>* __restrict__ sol, const double*
     comp += (fabs(sum) >= fabs(x)) ? (sum - t) + x : (x - t) + sum;
     sum = t;
   }
-  const double s0re = (double)sol[0].re, s0im = (double)sol[0].im;
+  double s0re = (double)sol[0].re, s0im = (double)sol[0].im;
+  if (sol_lo) {
+    s0re += (double)sol_lo[0].re;
+    s0im += (double)sol_lo[0].im;
+  }
   const double dc = s0re * s0re + s0im * s0im;
   rec[0] = s0re;
   rec[1] = ((sum - dc) + comp) / (npoints * npoints);

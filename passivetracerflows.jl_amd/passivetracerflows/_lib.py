@@ -125,6 +125,9 @@ SIGNATURES = {
     "ptf_ensemble_slot_name": [_VP, _I32, C.c_char_p, _I32],
     "ptf_ensemble_time_slot": [_VP, _I32, _I32, _DP],
     "ptf_ensemble_set_variant": [_VP, _I32, _I32, _I32, _I32],
+    "ptf_ensemble_get_clock": [_VP, _I32, _DP, C.POINTER(_I64)],
+    "ptf_ensemble_set_clock": [_VP, C.c_double, _I64],
+    "ptf_ensemble_diagnostics": [_VP, _I32, _DP, _DP],
     "ptf_ensemble_get_variant": [_VP, _I32, C.POINTER(_I32), C.POINTER(_I32), C.POINTER(_I32)],
 }
 

@@ -15,10 +15,30 @@ from . import _lib
 from ._lib import call, dptr
 
 
+def _gridpoints2(n, L=2 * np.pi):
+    start = np.longdouble(-L / 2)
+    step = np.longdouble(L / n)
+    x = (start + np.arange(n, dtype=np.longdouble) * step).astype(np.float64)
+    Y, X = np.meshgrid(x, x, indexing="ij")
+    return X, Y
+
+
 class Ensemble:
-    def __init__(self, members, dt, precision="mixed", stepper="RK4"):
-        """members: iterable of dicts with n (grid size, nx = ny), kappa, optional eta, L."""
+    def __init__(self, members, dt, precision="mixed", stepper="RK4", **filterkw):
+        """members: iterable of dicts with n (grid size, nx = ny), kappa, optional eta, kappah,
+        nkappah, L, and optional ``flow`` = a function (x, y, t) -> (u, v) for a time-varying
+        flow (evaluated at the member's clock.t before every step, S:708-723); members without
+        one take a steady flow from set_flow.  stepper: RK4, ETDRK4, FilteredRK4 or
+        FilteredETDRK4 (filter keywords order, innerK, outerK, tol as FourierFlows')."""
         members = list(members)
+        st = stepper.upper()
+        if st not in _lib.STEPPERS:
+            raise ValueError(f"unknown stepper {stepper!r}")
+        unknown = set(filterkw) - {"order", "innerK", "outerK", "tol"}
+        if unknown or (filterkw and not st.startswith("FILTERED")):
+            raise TypeError(f"unexpected keywords {sorted(filterkw)}")
+        self._tv = {}
+        self._L = []
         arr = (_lib.ptf_desc * len(members))()
         self.sizes = []
         for i, m in enumerate(members):
@@ -34,10 +54,20 @@ class Ensemble:
             d.iota = d.kappa
             d.kappah = float(m.get("kappah", 0.0))
             d.nkappah = int(m.get("nkappah", 0))
-            d.stepper = _lib.PTF_RK4 if stepper.upper() == "RK4" else _lib.PTF_ETDRK4
+            d.stepper = _lib.STEPPERS[st]
             d.dt = float(dt)
-            d.flow_kind = _lib.PTF_FLOW_STEADY
+            if st.startswith("FILTERED"):
+                fk = {**dict(order=4, innerK=0.65, outerK=1.0, tol=1e-15), **filterkw}
+                d.filter_order = int(fk["order"])
+                d.filter_innerK, d.filter_outerK, d.filter_tol = (float(fk["innerK"]),
+                                                                  float(fk["outerK"]), float(fk["tol"]))
+            if callable(m.get("flow")):
+                d.flow_kind = _lib.PTF_FLOW_PER_STEP
+                self._tv[i] = m["flow"]
+            else:
+                d.flow_kind = _lib.PTF_FLOW_STEADY
             self.sizes.append(n)
+            self._L.append(L)
         self.precision = "mixed" if precision == "mixed" else "fp64"
         self.stepper = stepper.upper()
         h = C.c_void_p()
@@ -68,8 +98,36 @@ class Ensemble:
         return out
 
     def step(self, nsteps=1, graph=True):
-        call("ptf_ensemble_step", self._h, int(nsteps), 1 if graph else 0)
+        """Advance every member nsteps steps.  Members with a time-varying flow get it
+        evaluated at their clock.t and uploaded before each step (then the ensemble steps
+        one step at a time)."""
+        if not self._tv:
+            call("ptf_ensemble_step", self._h, int(nsteps), 1 if graph else 0)
+        else:
+            for _ in range(int(nsteps)):
+                for i, f in self._tv.items():
+                    t, _ = self.clock(i)
+                    n = self.sizes[i]
+                    X, Y = _gridpoints2(n, self._L[i])
+                    u, v = f(X, Y, t)
+                    self.set_flow(i, np.broadcast_to(u, (n, n)), np.broadcast_to(v, (n, n)))
+                call("ptf_ensemble_step", self._h, 1, 1 if graph else 0)
         self.steps += int(nsteps)
+
+    def clock(self, i):
+        """(t, step) of member i: t advances by the member's dt after every step."""
+        t, s = C.c_double(), C.c_int64()
+        call("ptf_ensemble_get_clock", self._h, i, C.byref(t), C.byref(s))
+        return t.value, s.value
+
+    def set_clock(self, t=0.0, step=0):
+        call("ptf_ensemble_set_clock", self._h, float(t), int(step))
+
+    def diagnostics(self, i):
+        """FP64 (mass, variance) of member i."""
+        m, v = C.c_double(), C.c_double()
+        call("ptf_ensemble_diagnostics", self._h, i, C.byref(m), C.byref(v))
+        return m.value, v.value
 
     def set_stream(self, hip_stream):
         call("ptf_ensemble_set_stream", self._h, hip_stream)

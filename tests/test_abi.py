@@ -77,7 +77,8 @@ def _desc(**kw):
 @pytest.mark.parametrize("kw,code,msg", [
     (dict(ndim=4), _lib.PTF_ERR_INVALID, "ndim"),
     (dict(n=(63, 64, 1)), _lib.PTF_ERR_INVALID, "even"),
-    (dict(n=(96, 64, 1)), _lib.PTF_ERR_UNSUPPORTED, "power of two"),
+    (dict(n=(98, 64, 1)), _lib.PTF_ERR_UNSUPPORTED, "no compiled transform"),
+    (dict(n=(64, 2002, 1)), _lib.PTF_ERR_UNSUPPORTED, "no compiled transform"),   # 2 * 7 * 11 * 13
     (dict(n=(8, 64, 1)), _lib.PTF_ERR_UNSUPPORTED, "nx must be"),
     (dict(n=(64, 4096, 1)), _lib.PTF_ERR_UNSUPPORTED, "ny and nz"),
     (dict(dt=0.0), _lib.PTF_ERR_INVALID, "dt"),
@@ -99,6 +100,21 @@ def test_problem_create_rejects(kw, code, msg):
     assert msg in lib.ptf_last_error().decode()
 
 
+@pytest.mark.parametrize("n", [(96, 192, 1), (384, 96, 1), (640, 1000, 1), (1000, 64, 1)])
+def test_problem_create_accepts_mixed_radix_sizes(n):
+    """2^a 3^b 5^c sizes pass validation (on a machine without a GPU the creation then fails
+    at the first device call, never with PTF_ERR_UNSUPPORTED or PTF_ERR_INVALID)."""
+    lib = _lib.load()
+    for prec in (_lib.PTF_FP64, _lib.PTF_MIXED):
+        d = _desc(n=n, precision=prec)
+        h = C.c_void_p()
+        rc = lib.ptf_problem_create(C.byref(d), C.byref(h))
+        msg = lib.ptf_last_error().decode()
+        assert rc not in (_lib.PTF_ERR_UNSUPPORTED, _lib.PTF_ERR_INVALID), msg
+        if rc == _lib.PTF_OK:
+            lib.ptf_problem_destroy(h)
+
+
 def test_null_handles_rejected():
     lib = _lib.load()
     assert lib.ptf_step(None, 1) == _lib.PTF_ERR_INVALID
@@ -109,7 +125,7 @@ def test_null_handles_rejected():
     assert lib.ptf_ensemble_step(None, 1, 1) == _lib.PTF_ERR_INVALID
     h = C.c_void_p()
     assert lib.ptf_plan_create(2, None, 0, C.byref(h)) == _lib.PTF_ERR_INVALID
-    n = (C.c_int64 * 3)(30, 30, 1)
+    n = (C.c_int64 * 3)(42, 42, 1)      # 42 = 2 * 3 * 7: no compiled transform
     assert lib.ptf_plan_create(2, n, 0, C.byref(h)) == _lib.PTF_ERR_UNSUPPORTED
 
 

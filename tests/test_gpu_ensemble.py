@@ -88,3 +88,46 @@ def test_ensemble_member_equals_single_problem(dev):
         TAD.set_c_(p, m["c0"])
         ptf.stepforward_(p, nsteps)
         assert np.array_equal(p.sol, e.get_sol(i))
+
+
+def test_ensemble_filtered_timevarying_clock_and_diagnostics(dev):
+    """Members with FilteredRK4 (custom filter), a time-varying flow evaluated at each
+    member's clock.t, and per-member clocks and FP64 diagnostics, against the oracle."""
+    dt, nsteps = 5e-3, 12
+    base = _members()[:4]
+
+    def moving(U):
+        # a uniformly translating cellular flow: u(x, y, t) = U cos(t) + cellular
+        return lambda X, Y, t: (U * np.cos(t) - np.sin(X) * np.cos(Y), np.cos(X) * np.sin(Y))
+    members = []
+    for k, m in enumerate(base):
+        mm = dict(n=m["n"], kappa=m["kappa"])
+        if k % 2:
+            mm["flow"] = moving(0.3 + 0.1 * k)
+        members.append(mm)
+    filt = dict(order=6, innerK=0.5, outerK=1.0, tol=1e-12)
+    e = ptf.Ensemble(members, dt=dt, precision="fp64", stepper="FilteredRK4", **filt)
+    for i, m in enumerate(base):
+        if i % 2 == 0:
+            e.set_flow(i, *m["flow"])
+        e.set_c(i, m["c0"])
+    e.step(nsteps, graph=True)
+    for i, m in enumerate(base):
+        g = O.Grid(2, m["n"])
+        if i % 2:
+            f = members[i]["flow"]
+            X, Y = g.gridpoints()
+            flow = (lambda t, f=f, X=X, Y=Y: f(X, Y, t))
+        else:
+            flow = m["flow"]
+        o = O.Problem(g, flow, kappa=m["kappa"], dt=dt, stepper="FilteredRK4", **filt)
+        o.set_c(m["c0"])
+        o.stepforward(nsteps)
+        assert rel(e.get_sol(i), o.sol) <= 1e-12, i
+        t, s = e.clock(i)
+        assert s == nsteps and t == o.clock.t
+        mass, var = e.diagnostics(i)
+        assert abs(mass - O.mass(g, o.sol)) <= 1e-12 * np.abs(o.sol).max()
+        assert abs(var - O.variance(g, o.sol)) <= 1e-12 * O.variance(g, o.sol)
+    e.set_clock(0.0, 0)
+    assert e.clock(0) == (0.0, 0)
