@@ -133,6 +133,15 @@ int ptf_rfft(ptf_plan* plan, const void* in_real, void* out_complex);
  * superset of FFTW's destroy-input contract); only the real parts of the
  * kx = 0 and kx = nx/2 entries are used after the y/z inverse passes.        */
 int ptf_irfft(ptf_plan* plan, const void* in_complex, void* out_real);
+/* ptf_irfft without any device allocation by the library (no counterpart in the
+ * reference: for callers that cannot allocate, such as code under HIP graph capture,
+ * serving loops and harnesses that forbid hipMalloc).  The intermediate spectrum of a
+ * 2D/3D inverse goes into a static 64 MiB device array of the library (one per device)
+ * instead of the plan's scratch buffer: PTF_ERR_INVALID when the plan's batch of half
+ * spectra needs more (e.g. 3D 256^3 FP64).  Results are identical to ptf_irfft.  Every
+ * call on a device shares that device's array and it keeps the last call's contents:
+ * calls that overlap on one device (other streams, other threads) race. */
+int ptf_irfft_noalloc(ptf_plan* plan, const void* in_complex, void* out_real);
 /* dealias!(fh, grid) ([FF], used by the MQG-coupled path S:225-250): zero the
  * aliased wavenumbers of a device half spectrum of the plan's grid, kr indices
  * from floor((1-f)/2 nx) (0-based) to the end, l and m indices in

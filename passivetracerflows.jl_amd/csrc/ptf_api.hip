@@ -18,6 +18,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1004,22 +1005,26 @@ static int group_rfft(Group& g, const void* in_real, void* out_complex, hipStrea
   return PTF_OK;
 }
 
+// The inverse transforms along z and y write the intermediate spectrum into `spec` (the
+// plan's own scratch, or the static scratch of ptf_irfft_noalloc); the input is preserved.
 template <typename T>
-static int group_irfft(Group& g, const void* in_complex, void* out_real, hipStream_t st) {
+static int group_irfft(Group& g, const void* in_complex, void* out_real, hipStream_t st,
+                       void* spec = nullptr) {
+  cpx<T>* scr = (cpx<T>*)(spec ? spec : g.spec.p);
   const cpx<T>* src = (const cpx<T>*)in_complex;
   if (g.ndim == 3) {
     ColArgs<T>& a = args_of<ColArgs<T>>(g.p_zi);
     a.in0 = src;
-    a.out0 = (cpx<T>*)g.spec.p;
+    a.out0 = scr;
     CHK(launch(g.p_zi, st));
-    src = (const cpx<T>*)g.spec.p;
+    src = scr;
   }
   if (g.ndim >= 2) {
     ColArgs<T>& a = args_of<ColArgs<T>>(g.p_yi);
     a.in0 = src;
-    a.out0 = (cpx<T>*)g.spec.p;
+    a.out0 = scr;
     CHK(launch(g.p_yi, st));
-    src = (const cpx<T>*)g.spec.p;
+    src = scr;
   }
   RowArgs<T>& a = args_of<RowArgs<T>>(g.p_c2r);
   a.in[0] = src;
@@ -1031,8 +1036,43 @@ static int group_irfft(Group& g, const void* in_complex, void* out_real, hipStre
 static int g_rfft(Group& g, const void* in, void* out, hipStream_t st) {
   return g.dbl ? group_rfft<double>(g, in, out, st) : group_rfft<float>(g, in, out, st);
 }
-static int g_irfft(Group& g, const void* in, void* out, hipStream_t st) {
-  return g.dbl ? group_irfft<double>(g, in, out, st) : group_irfft<float>(g, in, out, st);
+static int g_irfft(Group& g, const void* in, void* out, hipStream_t st, void* spec = nullptr) {
+  return g.dbl ? group_irfft<double>(g, in, out, st, spec) : group_irfft<float>(g, in, out, st, spec);
+}
+
+// ---------------------------------------------------------------------------
+// static scratch of the allocation-free inverse transform (ptf_irfft_noalloc)
+// ---------------------------------------------------------------------------
+// Callers that cannot allocate device memory (code under HIP graph capture, serving
+// loops, harnesses that forbid hipMalloc) get the intermediate spectrum of a 2D/3D inverse
+// transform in this fixed array instead of the plan's scratch buffer.  Every device the
+// library runs on holds its own copy until the process ends.  In host code the name
+// `scratch` is not a device address: the address is taken once per device with
+// hipGetSymbolAddress (with that device current) and only that address is used.
+static constexpr size_t kStaticScratchBytes = size_t(64) << 20;   // 64 MiB
+static __device__ __align__(256) unsigned char scratch[kStaticScratchBytes];
+static constexpr int kMaxDevices = 64;
+static void* g_scratch_addr[kMaxDevices];
+static std::mutex g_scratch_mu;
+
+// device address of `scratch` on device `dev` (fetched on first use with `dev` current)
+static int static_scratch(int dev, void** out) {
+  if (dev < 0 || dev >= kMaxDevices)
+    return fail(PTF_ERR_INVALID, "device " + std::to_string(dev) + " out of range for the static scratch");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_addr[dev]) {
+    int cur = 0;
+    HIPCHK(hipGetDevice(&cur));
+    if (cur != dev) HIPCHK(hipSetDevice(dev));
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (cur != dev) HIPCHK(hipSetDevice(cur));
+    if (e != hipSuccess || !p)
+      return fail(PTF_ERR_DEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_addr[dev] = p;
+  }
+  *out = g_scratch_addr[dev];
+  return PTF_OK;
 }
 
 // time factor T(omega t + phase) of an analytic flow term
@@ -1276,6 +1316,7 @@ struct ptf_plan {
   Group g;
   hipStream_t stream = nullptr;
   bool own_stream = false;
+  int device = 0;   // the device current at creation (holds the plan's buffers and stream)
   ~ptf_plan() {
     if (own_stream && stream) (void)hipStreamDestroy(stream);
   }
@@ -1379,6 +1420,8 @@ int ptf_device_init(int32_t ordinal) {
     return fail(PTF_ERR_INVALID, "device ordinal " + std::to_string(ordinal) + " out of range (" +
                                      std::to_string(n) + " devices)");
   HIPCHK(hipSetDevice(ordinal));
+  void* scr = nullptr;
+  CHK(static_scratch(ordinal, &scr));   // the allocation-free entry point's scratch address
   return PTF_OK;
 }
 
@@ -1437,6 +1480,7 @@ int ptf_plan_create_batched(int32_t ndim, const int64_t n[3], int64_t batch, int
   d.dt = 1.0;
   CHK(validate_desc(&d));
   std::unique_ptr<ptf_plan> p(new ptf_plan());
+  HIPCHK(hipGetDevice(&p->device));
   std::vector<ptf_desc> ds((size_t)batch, d);
   CHK(group_init(p->g, ds.data(), (int)batch, /*allow_brick=*/false, /*plan_only=*/true));
   HIPCHK(hipStreamCreate(&p->stream));
@@ -1467,6 +1511,21 @@ int ptf_rfft(ptf_plan* plan, const void* in_real, void* out_complex) {
 int ptf_irfft(ptf_plan* plan, const void* in_complex, void* out_real) {
   if (!plan || !in_complex || !out_real) return fail(PTF_ERR_INVALID, "NULL argument");
   CHK(g_irfft(plan->g, in_complex, out_real, plan->stream));
+  return PTF_OK;
+}
+
+int ptf_irfft_noalloc(ptf_plan* plan, const void* in_complex, void* out_real) {
+  if (!plan || !in_complex || !out_real) return fail(PTF_ERR_INVALID, "NULL argument");
+  const Group& g = plan->g;
+  // 2D/3D: one intermediate spectrum of every member; 1D needs none
+  const size_t need = g.ndim >= 2 ? (size_t)g.nS * (size_t)g.B * 2 * g.esz : 0;
+  if (need > kStaticScratchBytes)
+    return fail(PTF_ERR_INVALID, "ptf_irfft_noalloc: the plan needs " + std::to_string(need) +
+                                     " bytes of scratch; the static scratch holds " +
+                                     std::to_string(kStaticScratchBytes) + " (use ptf_irfft)");
+  void* scr = nullptr;
+  if (need) CHK(static_scratch(plan->device, &scr));
+  CHK(g_irfft(plan->g, in_complex, out_real, plan->stream, scr));
   return PTF_OK;
 }
 

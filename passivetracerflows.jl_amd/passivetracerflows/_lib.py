@@ -83,6 +83,7 @@ SIGNATURES = {
     "ptf_plan_set_stream": [_VP, _VP],
     "ptf_rfft": [_VP, _VP, _VP],
     "ptf_irfft": [_VP, _VP, _VP],
+    "ptf_irfft_noalloc": [_VP, _VP, _VP],
     "ptf_problem_create": [C.POINTER(ptf_desc), C.POINTER(_VP)],
     "ptf_problem_destroy": [_VP],
     "ptf_problem_set_stream": [_VP, _VP],
