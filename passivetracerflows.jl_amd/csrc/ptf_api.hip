@@ -689,6 +689,12 @@ static int build_calcN(Group& g, const cpx<T>* in, const StageArgs<T>& st, cpx<T
 
 template <typename T>
 static int build_group_slots(Group& g) {
+  // Stage-boundary fusion (2D/3D): the last kernel of a stage also performs the first inverse
+  // pass of the next stage's input.  PTF_STAGE_FUSION=0 builds the unfused schedule (each
+  // stage starts with its own inverse pass, which reads the stored stage input), to measure
+  // the two against each other.
+  const char* fz = std::getenv("PTF_STAGE_FUSION");
+  const bool fuse = g.ndim >= 2 && !(fz && fz[0] == '0');
   for (int s = 0; s < 4; ++s) {
     g.stages[s].clear();
     const cpx<T>* in;
@@ -698,13 +704,13 @@ static int build_group_slots(Group& g) {
       in = (const cpx<T>*)(s == 0 ? g.sol.p : s == 1 ? g.stg.p : g.stg2.p);
     }
     StageArgs<T> st = stage_args<T>(g, s);
-    // the fused forms (ndim >= 2) hand the next stage input on in registers
-    st.store_x = g.ndim >= 2 ? 0 : 1;
+    // the fused forms hand the next stage input on in registers
+    st.store_x = fuse ? 0 : 1;
     CHK(build_calcN<T>(g, in, st, nullptr, g.stages[s], "stage" + std::to_string(s + 1) + "_",
-                       g.ndim >= 2));
+                       fuse));
   }
   g.prologue.clear();
-  if (g.ndim >= 2) {
+  if (fuse) {
     const int axis = g.ndim == 3 ? 2 : 1;
     Slot s1 = make_slot(g, K_COL_D1, axis, g.B);
     ColArgs<T> a1 = col_base<T>(g, axis);
@@ -1227,9 +1233,22 @@ static std::string tune_key(const Slot& s) {
          ":" + std::to_string(s.members);
 }
 
+// In-step durations: `nsteps` whole steps of the group with an event after every launch
+// (after one unmeasured step); out[key] = the summed duration per step of the launches of
+// each tune key.  The state advances (callers restore it).
+static int instep_times(Group& g, hipStream_t st, int nsteps,
+                        std::vector<std::pair<std::string, double>>* out);
+
+// one candidate of a tune key: (R, W, occ, par) with its isolated-launch score
+struct TuneCand {
+  int R, W, occ, par;
+  double ms;
+};
+
 static int autotune_group(Group& g, hipStream_t st) {
   std::vector<Slot*> all = step_slots(g);
   std::vector<std::string> done;
+  std::vector<std::pair<std::string, std::vector<TuneCand>>> shortlist;
   for (Slot* s : all) {
     const std::string key = tune_key(*s);
     if (std::find(done.begin(), done.end(), key) != done.end()) continue;
@@ -1247,6 +1266,7 @@ static int autotune_group(Group& g, hipStream_t st) {
     double best = 1e30;
     const bool verbose = std::getenv("PTF_AUTOTUNE_VERBOSE") != nullptr;
     int bestPar = s->par;
+    std::vector<TuneCand> cands;
     for (int par : {0, 1}) {
       for (int occ : {0, 3}) {
         for (int R : kRadices) {
@@ -1267,6 +1287,7 @@ static int autotune_group(Group& g, hipStream_t st) {
             if (verbose)
               std::fprintf(stderr, "autotune %s N=%d R=%d W=%d occ=%d par=%d: %.4f ms (%d launches)\n",
                            kind_name(s->kind), s->N, R, widths[wi], occ, par, tot, (int)same.size());
+            cands.push_back({R, widths[wi], occ, par, tot});
             if (tot < best) {
               best = tot;
               bestR = R;
@@ -1280,8 +1301,105 @@ static int autotune_group(Group& g, hipStream_t st) {
     }
     for (Slot* o : all)
       if (tune_key(*o) == key) choose_variant(*o, bestR, bestW, bestOcc, bestPar);
+    std::sort(cands.begin(), cands.end(), [](const TuneCand& a, const TuneCand& b) { return a.ms < b.ms; });
+    if (cands.size() > 4) cands.resize(4);
+    shortlist.push_back({key, cands});
+  }
+  // Second pass, inside running steps: a launch timed alone re-reads data that its previous
+  // repetition left in the caches, which favours some variants; here each key's shortlist
+  // (the four fastest alone) is timed in whole steps, where every kernel reads what the
+  // previous one wrote, and the fastest there is kept.  The state is restored afterwards.
+  if (std::getenv("PTF_AUTOTUNE_ISOLATED") == nullptr && !g.plan_only && g.sol.p) {
+    const size_t S = (size_t)g.nS * g.B * 2 * g.esz;
+    if (!g.backup.p) CHK(dalloc(g.backup, 2 * S));
+    HIPCHK(hipMemcpyAsync(g.backup.p, g.sol.p, S, hipMemcpyDeviceToDevice, st));
+    if (g.sol_lo.p)
+      HIPCHK(hipMemcpyAsync((char*)g.backup.p + S, g.sol_lo.p, S, hipMemcpyDeviceToDevice, st));
+    const bool verbose = std::getenv("PTF_AUTOTUNE_VERBOSE") != nullptr;
+    for (auto& ent : shortlist) {
+      const std::string& key = ent.first;
+      if (ent.second.size() < 2) continue;
+      double best = 1e30;
+      TuneCand pick = ent.second[0];
+      for (const TuneCand& c : ent.second) {
+        for (Slot* o : all)
+          if (tune_key(*o) == key) choose_variant(*o, c.R, c.W, c.occ, c.par);
+        std::vector<std::pair<std::string, double>> t;
+        CHK(instep_times(g, st, 3, &t));
+        double ms = 0;
+        for (auto& kv : t)
+          if (kv.first == key) ms = kv.second;
+        if (verbose)
+          std::fprintf(stderr, "autotune in-step %s R=%d W=%d occ=%d par=%d: %.4f ms/step (alone %.4f)\n",
+                       key.c_str(), c.R, c.W, c.occ, c.par, ms, c.ms);
+        if (ms < best) {
+          best = ms;
+          pick = c;
+        }
+      }
+      for (Slot* o : all)
+        if (tune_key(*o) == key) choose_variant(*o, pick.R, pick.W, pick.occ, pick.par);
+    }
+    HIPCHK(hipMemcpyAsync(g.sol.p, g.backup.p, S, hipMemcpyDeviceToDevice, st));
+    if (g.sol_lo.p)
+      HIPCHK(hipMemcpyAsync(g.sol_lo.p, (char*)g.backup.p + S, S, hipMemcpyDeviceToDevice, st));
+    HIPCHK(hipStreamSynchronize(st));
+    g.primed = false;
   }
   return PTF_OK;
+}
+
+static int instep_times(Group& g, hipStream_t st, int nsteps,
+                        std::vector<std::pair<std::string, double>>* out) {
+  const std::vector<Slot*> v = step_slots(g);
+  const size_t ns = v.size();
+  g.primed = false;
+  CHK(group_step_launches(g, st, 0.0));   // one unmeasured step (also primes)
+  std::vector<hipEvent_t> ev((size_t)nsteps * ns + 1, nullptr);
+  int rc = PTF_OK;
+  for (auto& e : ev) {
+    if (hipEventCreate(&e) != hipSuccess) {
+      e = nullptr;
+      rc = fail(PTF_ERR_DEVICE, "hipEventCreate failed");
+      break;
+    }
+  }
+  size_t k = 0;
+  if (rc == PTF_OK && hipEventRecord(ev[k++], st) != hipSuccess) rc = fail(PTF_ERR_DEVICE, "hipEventRecord");
+  for (int i = 0; i < nsteps && rc == PTF_OK; ++i) {
+    patch_flowsyn(g, 0.0);
+    for (Slot* s : v) {
+      if ((rc = launch(*s, st)) != PTF_OK) break;
+      if (hipEventRecord(ev[k++], st) != hipSuccess) {
+        rc = fail(PTF_ERR_DEVICE, "hipEventRecord");
+        break;
+      }
+    }
+  }
+  if (rc == PTF_OK && hipEventSynchronize(ev[k - 1]) != hipSuccess)
+    rc = fail(PTF_ERR_DEVICE, "hipEventSynchronize");
+  if (rc == PTF_OK) {
+    out->clear();
+    for (size_t e = 1; e < k && rc == PTF_OK; ++e) {
+      float ms = 0;
+      if (hipEventElapsedTime(&ms, ev[e - 1], ev[e]) != hipSuccess) {
+        rc = fail(PTF_ERR_DEVICE, "hipEventElapsedTime");
+        break;
+      }
+      const std::string key = tune_key(*v[(e - 1) % ns]);
+      bool found = false;
+      for (auto& kv : *out)
+        if (kv.first == key) {
+          kv.second += ms / nsteps;
+          found = true;
+        }
+      if (!found) out->push_back({key, ms / nsteps});
+    }
+  }
+  for (auto& e : ev)
+    if (e) (void)hipEventDestroy(e);
+  g.primed = false;
+  return rc;
 }
 
 // ---------------------------------------------------------------------------
