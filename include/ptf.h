@@ -218,8 +218,13 @@ int ptf_problem_slot_name(ptf_problem* prob, int32_t slot, char* buf, int32_t bu
 int ptf_problem_time_slot(ptf_problem* prob, int32_t slot, int32_t reps, double* avg_ms);
 /* Kernel variant of slot i: radix R, width W (columns or rows per block) and
  * flags (PTF_VARIANT_*).  set_variant selects any compiled variant (as the
- * autotuner does); PTF_ERR_UNSUPPORTED when that variant was not compiled. */
-enum { PTF_VARIANT_OCC3 = 1, PTF_VARIANT_PAR = 2 };
+ * autotuner does); PTF_ERR_UNSUPPORTED when that variant was not compiled.
+ * OCC3: a 3-waves/SIMD register budget; PAR: the input-parallel advection form
+ * (row slots) or the persistent pipelined stage kernel (fused column slots);
+ * WG_LINES: a column line along the bricked axis spread over the workgroup with
+ * workgroup barriers, where by default it is held by one wavefront (accepted only
+ * for the slots and radices where the per-wavefront form applies). */
+enum { PTF_VARIANT_OCC3 = 1, PTF_VARIANT_PAR = 2, PTF_VARIANT_WG_LINES = 4 };
 int ptf_problem_set_variant(ptf_problem* prob, int32_t slot, int32_t R, int32_t W, int32_t flags);
 int ptf_problem_get_variant(ptf_problem* prob, int32_t slot, int32_t* R, int32_t* W, int32_t* flags);
 /* ptf_step with a HIP event recorded around every launch of every step:

@@ -14,6 +14,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstddef>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -162,6 +163,8 @@ struct Slot {
   int R = 0, W = 0;     // radix, columns/rows per block
   int occ = 0;          // 0: default register budget; 3: 3 waves/SIMD
   int par = 0;          // 1: input-parallel advection (row_adv_par_kernel)
+  int wg = 0;           // 1: a bricked column line spread over the workgroup (workgroup
+                        // barriers) where it would fit one wavefront (see wave_ok)
   const void* fn = nullptr;
   long long lines = 0;  // ncols (columns) or nrows (rows)
   int other = 1;        // grid.y for columns
@@ -173,6 +176,16 @@ struct Slot {
   unsigned fixed_blocks = 0;
 
   bool is_col() const { return kind < 10; }
+  // Per-wavefront lines apply to a column kernel along a bricked axis whose P = N/R threads
+  // per line divide a wavefront (FP32: at most 32, so that a wavefront still reads whole
+  // 128-byte runs); the same test as kWaveOk in col_kernel.  The persistent pipelined
+  // kernel keeps its own lane mapping.
+  bool wave_ok(int r) const {
+    if (!is_col() || brick <= 0 || fixed_threads > 0 || r <= 0 || (kind == 6 && par)) return false;
+    const int P = N / r;
+    return P <= 64 && 64 % P == 0 && (dbl || P <= 32);
+  }
+  int wave_lines() const { return wave_ok(R) && !wg ? 1 : 0; }
   bool has_occ() const { return kind == 3 || kind == 6 || kind == 13 || kind == 14; }
   // par: the input-parallel advection form (kinds 13, 14) or the pipelined persistent
   // stage-boundary kernel (kind 6)
@@ -196,10 +209,15 @@ struct Slot {
   }
 };
 
-static bool choose_variant(Slot& s, int R, int W, int occ = 0, int par = 0) {
+static bool choose_variant(Slot& s, int R, int W, int occ = 0, int par = 0, int wg = 0) {
   if (!kind_known(s.kind)) return false;
   if (occ && !s.has_occ()) return false;
   if (par && !s.has_par()) return false;
+  if (wg) {
+    Slot t = s;
+    t.par = par;
+    if (!t.wave_ok(R)) return false;   // the workgroup form only where the wavefront one applies
+  }
   if (s.brick > 0) {
     // a bricked line needs whole bricks per thread group; bricked rows go in whole bricks
     // or in power-of-two parts of one brick per block (every row width divides or is a
@@ -213,6 +231,7 @@ static bool choose_variant(Slot& s, int R, int W, int occ = 0, int par = 0) {
   s.W = W;
   s.occ = occ;
   s.par = par;
+  s.wg = wg;
   s.fn = p;
   return true;
 }
@@ -236,7 +255,21 @@ static int pick_default(Slot& s) {
                                         " with line length " + std::to_string(s.N));
 }
 
+static_assert(sizeof(ColArgs<float>) == sizeof(ColArgs<double>) &&
+                  offsetof(ColArgs<float>, wave_lines) == offsetof(ColArgs<double>, wave_lines),
+              "column arguments share one layout");
+
 static int launch(const Slot& s, hipStream_t stream) {
+  if (s.is_col() && s.fixed_threads == 0 && s.args.size() == sizeof(ColArgs<double>)) {
+    // the lane mapping of the selected variant (the kernel arguments are copied at launch)
+    alignas(16) unsigned char buf[sizeof(ColArgs<double>)];
+    std::memcpy(buf, s.args.data(), sizeof(buf));
+    const int wl = s.wave_lines();
+    std::memcpy(buf + offsetof(ColArgs<double>, wave_lines), &wl, sizeof(wl));
+    void* argv[1] = {(void*)buf};
+    HIPCHK(hipLaunchKernel(s.fn, s.grid(), dim3(s.threads()), argv, 0, stream));
+    return PTF_OK;
+  }
   void* argv[1] = {(void*)s.args.data()};
   HIPCHK(hipLaunchKernel(s.fn, s.grid(), dim3(s.threads()), argv, 0, stream));
   return PTF_OK;
@@ -470,6 +503,7 @@ static ColArgs<T> col_base(const Group& g, int axis) {
   // blockIdx.y extent of this pass (see make_slot) and the member count
   a.nother = g.ndim == 3 ? (axis == 1 ? (g.nz >> g.lb) : g.ny) : 1;
   a.nmembers = g.B;
+  a.wave_lines = 0;   // set per launch from the slot's variant
   return a;
 }
 
@@ -1239,9 +1273,9 @@ static std::string tune_key(const Slot& s) {
 static int instep_times(Group& g, hipStream_t st, int nsteps,
                         std::vector<std::pair<std::string, double>>* out);
 
-// one candidate of a tune key: (R, W, occ, par) with its isolated-launch score
+// one candidate of a tune key: (R, W, occ, par, wg) with its isolated-launch score
 struct TuneCand {
-  int R, W, occ, par;
+  int R, W, occ, par, wg;
   double ms;
 };
 
@@ -1262,45 +1296,40 @@ static int autotune_group(Group& g, hipStream_t st) {
     }
     const int* widths = s->is_col() ? kColWidths : kRowWidths;
     const int nw = s->is_col() ? kNumColWidths : kNumRowWidths;
-    int bestR = s->R, bestW = s->W, bestOcc = s->occ;
-    double best = 1e30;
+    TuneCand bestc = {s->R, s->W, s->occ, s->par, s->wg, 1e30};
     const bool verbose = std::getenv("PTF_AUTOTUNE_VERBOSE") != nullptr;
-    int bestPar = s->par;
     std::vector<TuneCand> cands;
-    for (int par : {0, 1}) {
-      for (int occ : {0, 3}) {
-        for (int R : kRadices) {
-          for (int wi = 0; wi < nw; ++wi) {
-            double tot = 0;
-            bool ok = true;
-            for (Slot* o : same) {
-              Slot trial = *o;
-              if (!choose_variant(trial, R, widths[wi], occ, par)) {
-                ok = false;
-                break;
+    for (int wg : {0, 1}) {
+      for (int par : {0, 1}) {
+        for (int occ : {0, 3}) {
+          for (int R : kRadices) {
+            for (int wi = 0; wi < nw; ++wi) {
+              double tot = 0;
+              bool ok = true;
+              for (Slot* o : same) {
+                Slot trial = *o;
+                if (!choose_variant(trial, R, widths[wi], occ, par, wg)) {
+                  ok = false;
+                  break;
+                }
+                double ms = 0;
+                CHK(time_slot(g, trial, 5, st, &ms));
+                tot += ms;
               }
-              double ms = 0;
-              CHK(time_slot(g, trial, 5, st, &ms));
-              tot += ms;
-            }
-            if (!ok) continue;
-            if (verbose)
-              std::fprintf(stderr, "autotune %s N=%d R=%d W=%d occ=%d par=%d: %.4f ms (%d launches)\n",
-                           kind_name(s->kind), s->N, R, widths[wi], occ, par, tot, (int)same.size());
-            cands.push_back({R, widths[wi], occ, par, tot});
-            if (tot < best) {
-              best = tot;
-              bestR = R;
-              bestW = widths[wi];
-              bestOcc = occ;
-              bestPar = par;
+              if (!ok) continue;
+              if (verbose)
+                std::fprintf(stderr, "autotune %s N=%d R=%d W=%d occ=%d par=%d wg=%d: %.4f ms (%d launches)\n",
+                             kind_name(s->kind), s->N, R, widths[wi], occ, par, wg, tot, (int)same.size());
+              const TuneCand c = {R, widths[wi], occ, par, wg, tot};
+              cands.push_back(c);
+              if (tot < bestc.ms) bestc = c;
             }
           }
         }
       }
     }
     for (Slot* o : all)
-      if (tune_key(*o) == key) choose_variant(*o, bestR, bestW, bestOcc, bestPar);
+      if (tune_key(*o) == key) choose_variant(*o, bestc.R, bestc.W, bestc.occ, bestc.par, bestc.wg);
     std::sort(cands.begin(), cands.end(), [](const TuneCand& a, const TuneCand& b) { return a.ms < b.ms; });
     if (cands.size() > 4) cands.resize(4);
     shortlist.push_back({key, cands});
@@ -1323,22 +1352,22 @@ static int autotune_group(Group& g, hipStream_t st) {
       TuneCand pick = ent.second[0];
       for (const TuneCand& c : ent.second) {
         for (Slot* o : all)
-          if (tune_key(*o) == key) choose_variant(*o, c.R, c.W, c.occ, c.par);
+          if (tune_key(*o) == key) choose_variant(*o, c.R, c.W, c.occ, c.par, c.wg);
         std::vector<std::pair<std::string, double>> t;
         CHK(instep_times(g, st, 3, &t));
         double ms = 0;
         for (auto& kv : t)
           if (kv.first == key) ms = kv.second;
         if (verbose)
-          std::fprintf(stderr, "autotune in-step %s R=%d W=%d occ=%d par=%d: %.4f ms/step (alone %.4f)\n",
-                       key.c_str(), c.R, c.W, c.occ, c.par, ms, c.ms);
+          std::fprintf(stderr, "autotune in-step %s R=%d W=%d occ=%d par=%d wg=%d: %.4f ms/step (alone %.4f)\n",
+                       key.c_str(), c.R, c.W, c.occ, c.par, c.wg, ms, c.ms);
         if (ms < best) {
           best = ms;
           pick = c;
         }
       }
       for (Slot* o : all)
-        if (tune_key(*o) == key) choose_variant(*o, pick.R, pick.W, pick.occ, pick.par);
+        if (tune_key(*o) == key) choose_variant(*o, pick.R, pick.W, pick.occ, pick.par, pick.wg);
     }
     HIPCHK(hipMemcpyAsync(g.sol.p, g.backup.p, S, hipMemcpyDeviceToDevice, st));
     if (g.sol_lo.p)
@@ -2177,9 +2206,9 @@ int ptf_problem_num_slots(ptf_problem* prob, int32_t* n) {
 static int slot_name_into(Slot& s, char* buf, int32_t buflen) {
   if (!buf || buflen <= 0) return fail(PTF_ERR_INVALID, "bad buffer");
   char tmp[256];
-  std::snprintf(tmp, sizeof(tmp), "%s [%s N=%d R=%d W=%d%s%s %s]", s.name.c_str(),
+  std::snprintf(tmp, sizeof(tmp), "%s [%s N=%d R=%d W=%d%s%s%s %s]", s.name.c_str(),
                 kind_name(s.kind), s.N, s.R, s.W, s.occ ? " occ=3" : "", s.par ? " par" : "",
-                s.dbl ? "f64" : "f32");
+                s.wg ? " wg" : "", s.dbl ? "f64" : "f32");
   std::strncpy(buf, tmp, (size_t)buflen - 1);
   buf[buflen - 1] = 0;
   return PTF_OK;
@@ -2203,10 +2232,11 @@ int ptf_problem_set_variant(ptf_problem* prob, int32_t slot, int32_t R, int32_t 
   if (!prob) return fail(PTF_ERR_INVALID, "problem is NULL");
   auto v = step_slots(prob->g);
   if (slot < 0 || slot >= (int)v.size()) return fail(PTF_ERR_INVALID, "slot out of range");
-  if (flags & ~(PTF_VARIANT_OCC3 | PTF_VARIANT_PAR)) return fail(PTF_ERR_INVALID, "unknown flags");
+  if (flags & ~(PTF_VARIANT_OCC3 | PTF_VARIANT_PAR | PTF_VARIANT_WG_LINES))
+    return fail(PTF_ERR_INVALID, "unknown flags");
   Slot trial = *v[slot];
   if (!choose_variant(trial, R, W, (flags & PTF_VARIANT_OCC3) ? 3 : 0,
-                      (flags & PTF_VARIANT_PAR) ? 1 : 0))
+                      (flags & PTF_VARIANT_PAR) ? 1 : 0, (flags & PTF_VARIANT_WG_LINES) ? 1 : 0))
     return fail(PTF_ERR_UNSUPPORTED, "no compiled variant R=" + std::to_string(R) +
                                          " W=" + std::to_string(W) + " flags=" +
                                          std::to_string(flags) + " for slot " + trial.name);
@@ -2221,7 +2251,8 @@ int ptf_problem_get_variant(ptf_problem* prob, int32_t slot, int32_t* R, int32_t
   if (slot < 0 || slot >= (int)v.size()) return fail(PTF_ERR_INVALID, "slot out of range");
   *R = v[slot]->R;
   *W = v[slot]->W;
-  *flags = (v[slot]->occ ? PTF_VARIANT_OCC3 : 0) | (v[slot]->par ? PTF_VARIANT_PAR : 0);
+  *flags = (v[slot]->occ ? PTF_VARIANT_OCC3 : 0) | (v[slot]->par ? PTF_VARIANT_PAR : 0) |
+           (v[slot]->wg ? PTF_VARIANT_WG_LINES : 0);
   return PTF_OK;
 }
 
@@ -2481,10 +2512,11 @@ int ptf_ensemble_set_variant(ptf_ensemble* ens, int32_t slot, int32_t R, int32_t
   if (!ens) return fail(PTF_ERR_INVALID, "ensemble is NULL");
   auto v = ens_slots(ens);
   if (slot < 0 || slot >= (int)v.size()) return fail(PTF_ERR_INVALID, "slot out of range");
-  if (flags & ~(PTF_VARIANT_OCC3 | PTF_VARIANT_PAR)) return fail(PTF_ERR_INVALID, "unknown flags");
+  if (flags & ~(PTF_VARIANT_OCC3 | PTF_VARIANT_PAR | PTF_VARIANT_WG_LINES))
+    return fail(PTF_ERR_INVALID, "unknown flags");
   Slot trial = *v[slot].second;
   if (!choose_variant(trial, R, W, (flags & PTF_VARIANT_OCC3) ? 3 : 0,
-                      (flags & PTF_VARIANT_PAR) ? 1 : 0))
+                      (flags & PTF_VARIANT_PAR) ? 1 : 0, (flags & PTF_VARIANT_WG_LINES) ? 1 : 0))
     return fail(PTF_ERR_UNSUPPORTED, "no compiled variant R=" + std::to_string(R) +
                                          " W=" + std::to_string(W) + " flags=" +
                                          std::to_string(flags) + " for slot " + trial.name);
@@ -2501,7 +2533,8 @@ int ptf_ensemble_get_variant(ptf_ensemble* ens, int32_t slot, int32_t* R, int32_
   const Slot& s = *v[slot].second;
   *R = s.R;
   *W = s.W;
-  *flags = (s.occ ? PTF_VARIANT_OCC3 : 0) | (s.par ? PTF_VARIANT_PAR : 0);
+  *flags = (s.occ ? PTF_VARIANT_OCC3 : 0) | (s.par ? PTF_VARIANT_PAR : 0) |
+           (s.wg ? PTF_VARIANT_WG_LINES : 0);
   return PTF_OK;
 }
 

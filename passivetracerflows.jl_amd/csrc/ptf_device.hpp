@@ -274,9 +274,23 @@ struct LayStrided {      // runtime strides: line-major (rows contiguous) or ele
 // One pass of a line FFT (see the header): the E/RP butterflies of this thread on its
 // natural-layout registers a (already read from LDS unless this is the first pass),
 // outputs stored to LDS at their Stockham positions unless this is the last pass.
+// Orders a block's LDS accesses around an exchange: a workgroup barrier, or, when every
+// line of the block lies within one wavefront (wave = true), only that wavefront's: a
+// wavefront's LDS operations complete in issue order, so the fences (which keep the
+// compiler from moving LDS accesses across) and the wavefront barrier suffice.
+__device__ __forceinline__ void lds_sync(bool wave) {
+  if (wave) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  } else {
+    __syncthreads();
+  }
+}
+
 template <typename T, int N, int E, int DIR, int PASS, class Lay>
 __device__ __forceinline__ void fft_pass(cpx<T> (&a)[E], cpx<T>* lds, const Lay& lay, int c, int j,
-                                         const cpx<T>* __restrict__ tw, int twstride) {
+                                         const cpx<T>* __restrict__ tw, int twstride, bool wave) {
   constexpr RadixPlan pl = radix_plan(N, E);
   constexpr int P = N / E;
   constexpr int RP = pl.r[PASS];
@@ -285,7 +299,7 @@ __device__ __forceinline__ void fft_pass(cpx<T> (&a)[E], cpx<T>* lds, const Lay&
   constexpr bool kLast = PASS == pl.n - 1;
   static_assert(E % RP == 0, "a radix must divide the elements per thread");
   if constexpr (PASS > 0) {
-    __syncthreads();
+    lds_sync(wave);
 #pragma unroll
     for (int q = 0; q < E; ++q) a[q] = lds[lay.idx(c, j + q * P)];
   }
@@ -309,7 +323,7 @@ __device__ __forceinline__ void fft_pass(cpx<T> (&a)[E], cpx<T>* lds, const Lay&
     for (int r = 0; r < RP; ++r) a[s + r * B] = t[r];
   }
   if constexpr (!kLast) {
-    __syncthreads();
+    lds_sync(wave);
 #pragma unroll
     for (int s = 0; s < B; ++s) {
       const int b = j + s * P;
@@ -322,11 +336,11 @@ __device__ __forceinline__ void fft_pass(cpx<T> (&a)[E], cpx<T>* lds, const Lay&
 
 template <typename T, int N, int E, int DIR, int PASS, class Lay>
 __device__ __forceinline__ void fft_passes(cpx<T> (&a)[E], cpx<T>* lds, const Lay& lay, int c, int j,
-                                           const cpx<T>* __restrict__ tw, int twstride) {
+                                           const cpx<T>* __restrict__ tw, int twstride, bool wave) {
   constexpr RadixPlan pl = radix_plan(N, E);
   if constexpr (PASS < pl.n) {
-    fft_pass<T, N, E, DIR, PASS, Lay>(a, lds, lay, c, j, tw, twstride);
-    fft_passes<T, N, E, DIR, PASS + 1, Lay>(a, lds, lay, c, j, tw, twstride);
+    fft_pass<T, N, E, DIR, PASS, Lay>(a, lds, lay, c, j, tw, twstride, wave);
+    fft_passes<T, N, E, DIR, PASS + 1, Lay>(a, lds, lay, c, j, tw, twstride, wave);
   }
 }
 
@@ -334,16 +348,19 @@ __device__ __forceinline__ void fft_passes(cpx<T> (&a)[E], cpx<T>* lds, const La
 // header).  tw[m] = exp(-2 pi i m / LT) with LT = N * twstride.  With more than one
 // pass it starts with a barrier before its first LDS write and ends after its last LDS
 // read (callers that write LDS afterwards must barrier first); every thread of the block
-// must call it.
+// must call it.  wave = true (every line of the block within one wavefront, lines never
+// sharing LDS entries) orders the exchanges per wavefront instead (lds_sync); then a
+// twiddle table in LDS must be complete before the call.
 template <typename T, int N, int R, int DIR, class Lay>
 __device__ __forceinline__ void line_fft(cpx<T> (&a)[R], cpx<T>* lds, const Lay& lay, int c, int j,
-                                         const cpx<T>* __restrict__ tw, int twstride) {
+                                         const cpx<T>* __restrict__ tw, int twstride,
+                                         bool wave = false) {
   constexpr RadixPlan pl = radix_plan(N, R);
   static_assert(pl.n > 0, "N must split into radices that divide R");
   if constexpr (pl.n == 1) {
     dft<R, DIR>(a);
   } else {
-    fft_passes<T, N, R, DIR, 0, Lay>(a, lds, lay, c, j, tw, twstride);
+    fft_passes<T, N, R, DIR, 0, Lay>(a, lds, lay, c, j, tw, twstride, wave);
   }
 }
 

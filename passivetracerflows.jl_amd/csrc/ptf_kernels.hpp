@@ -57,6 +57,9 @@ struct ColArgs {
   // pipelined kernel, which is launched on a 1D grid
   int nother;
   int nmembers;
+  // nonzero: a bricked line whose P threads divide a wavefront is held by lanes of one
+  // wavefront (see col_kernel), and its exchanges are ordered per wavefront
+  int wave_lines;
 };
 
 // Occupancy range for a block of `threads` threads using `lds` bytes of LDS:
@@ -114,25 +117,32 @@ void col_kernel(ColArgs<T> a) {
   __shared__ cpx<T> lds[(N > R) ? N * TX : 1];
   __shared__ cpx<T> twl[col_tw_lds<N, R>() ? N : 1];
   const int t = threadIdx.x;
+  const int lb = a.lb;
+  // Lines within one wavefront: along a bricked axis, when the P threads of a line divide
+  // a wavefront and a wavefront's 64 / P lines still cover whole 128-byte runs per load.
+  constexpr bool kWaveOk = P <= 64 && 64 % P == 0 && (sizeof(T) == 8 || P <= 32);
+  const bool wave = kWaveOk && lb > 0 && a.wave_lines != 0;
   if constexpr (col_tw_lds<N, R>()) {
     // twl[m] = exp(-2 pi i m / N); the first barrier of every transform orders these
-    // writes before its first twiddle read
+    // writes before its first twiddle read (with per-wavefront exchanges, this barrier)
     for (int i = t; i < N; i += TX * P) twl[i] = a.tw[i * a.twstride];
+    if (wave) __syncthreads();
   }
   // lane t = rb + 2^lb (c + TX jh): the brick's planes vary fastest, then the
   // columns, so a wavefront reads 2^lb * TX contiguous values; the line
-  // element of register q is i = j + q P with j = rb + 2^lb jh (P % 2^lb == 0)
-  const int lb = a.lb;
+  // element of register q is i = j + q P with j = rb + 2^lb jh (P % 2^lb == 0).
+  // Per-wavefront lines: lane t = rb + 2^lb (jh + (P / 2^lb) c), so the P lanes of a line
+  // are consecutive and a wavefront reads runs of 2^lb * 64 / P values.
   const int rb = t & ((1 << lb) - 1);
   const int rest = t >> lb;
-  const int c = rest % TX;
-  const int jh = rest / TX;
+  const int c = wave ? rest / (P >> lb) : rest % TX;
+  const int jh = wave ? rest % (P >> lb) : rest / TX;
   const int j = rb + (jh << lb);
   // the line transform in direction D with this block's twiddle source
   auto fft = [&](auto dir_c, cpx<T>(&x)[R]) {
     constexpr int D = decltype(dir_c)::value;
-    if constexpr (col_tw_lds<N, R>()) line_fft<T, N, R, D, Lay>(x, lds, lay, c, j, twl, 1);
-    else line_fft<T, N, R, D, Lay>(x, lds, lay, c, j, a.tw, a.twstride);
+    if constexpr (col_tw_lds<N, R>()) line_fft<T, N, R, D, Lay>(x, lds, lay, c, j, twl, 1, wave);
+    else line_fft<T, N, R, D, Lay>(x, lds, lay, c, j, a.tw, a.twstride, wave);
   };
   int bx, by, member;
   xcd_remap(bx, by, member);

@@ -13,6 +13,7 @@ import numpy as np
 
 from . import _lib
 from ._lib import call, dptr
+from .fourierflows import variant_flags
 
 
 def _gridpoints2(n, L=2 * np.pi):
@@ -147,13 +148,14 @@ class Ensemble:
         sizes = ",".join(f"{c}x{n}^2" for n, c in sorted(Counter(self.sizes).items()))
         return f"ensemble {sizes} {self.precision} {self.stepper}"
 
-    def set_variant(self, i, R, W, occ3=False, par=False):
-        call("ptf_ensemble_set_variant", self._h, i, R, W, (1 if occ3 else 0) | (2 if par else 0))
+    def set_variant(self, i, R, W, occ3=False, par=False, wg=False):
+        call("ptf_ensemble_set_variant", self._h, i, R, W, variant_flags(occ3, par, wg))
 
     def get_variant(self, i):
+        """(R, W, occ3, par, wg) of launch slot i."""
         R, W, f = C.c_int32(), C.c_int32(), C.c_int32()
         call("ptf_ensemble_get_variant", self._h, i, C.byref(R), C.byref(W), C.byref(f))
-        return R.value, W.value, bool(f.value & 1), bool(f.value & 2)
+        return R.value, W.value, bool(f.value & 1), bool(f.value & 2), bool(f.value & 4)
 
     def slots(self):
         n = C.c_int32()

@@ -187,6 +187,11 @@ class DeviceArray:
             pass
 
 
+def variant_flags(occ3=False, par=False, wg=False):
+    """PTF_VARIANT_* bits: OCC3 = 1, PAR = 2, WG_LINES = 4 (include/ptf.h)."""
+    return (1 if occ3 else 0) | (2 if par else 0) | (4 if wg else 0)
+
+
 def _devptr(a, shape=None, dtype=None, what="array"):
     """Device address of a DeviceArray after checking it against what the plan expects.
 
@@ -402,15 +407,17 @@ class Problem:
         call("ptf_problem_time_slot", self._h, i, reps, C.byref(ms))
         return ms.value
 
-    def set_variant(self, i, R, W, occ3=False, par=False):
+    def set_variant(self, i, R, W, occ3=False, par=False, wg=False):
         """Select kernel variant (radix R, width W, 3-waves/SIMD build, input-parallel
-        advection) for launch slot i; raises PTFError if it was not compiled."""
-        call("ptf_problem_set_variant", self._h, i, R, W, (1 if occ3 else 0) | (2 if par else 0))
+        advection, bricked column lines spread over the workgroup) for launch slot i;
+        raises PTFError if it was not compiled or does not apply."""
+        call("ptf_problem_set_variant", self._h, i, R, W, variant_flags(occ3, par, wg))
 
     def get_variant(self, i):
+        """(R, W, occ3, par, wg) of launch slot i."""
         R, W, f = C.c_int32(), C.c_int32(), C.c_int32()
         call("ptf_problem_get_variant", self._h, i, C.byref(R), C.byref(W), C.byref(f))
-        return R.value, W.value, bool(f.value & 1), bool(f.value & 2)
+        return R.value, W.value, bool(f.value & 1), bool(f.value & 2), bool(f.value & 4)
 
     def step_profiled(self, nsteps):
         """Advance nsteps steps (as stepforward_) with a HIP event around every launch;

@@ -5,7 +5,8 @@ problem's own buffers and keeps the fastest.  Its choice varies a little from ru
 where two variants are within noise of each other, so a benchmark that is profiled in a
 second process would otherwise time one variant and profile another.  A cache file maps a
 problem key (shape, precision, stepper, flow kind) to the per-slot (radix, width,
-3-waves/SIMD, input-parallel) choice; applying it needs no timing.
+3-waves/SIMD, input-parallel, workgroup-lines) choice; applying it needs no timing.
+Entries written before the workgroup-lines flag existed (five fields) read it as off.
 """
 from __future__ import annotations
 
@@ -34,15 +35,17 @@ def store(path, data):
 
 def apply_or_tune(obj, key, path, tune=None):
     """Apply the cached variants of ``key`` to ``obj`` (a Problem or an Ensemble: slots(),
-    set_variant(i, R, W, occ3, par), get_variant(i)); if there is no matching entry, run the
+    set_variant(i, R, W, occ3, par, wg), get_variant(i)); if there is no matching entry, run the
     autotuner (``tune``, default the object's own) and store its choice.  Returns "cache" or
     "tuned"."""
     data = load(path)
     names = [_base(n) for n in obj.slots()]
     entry = data.get(key)
     if entry and [e[0] for e in entry] == names:
-        for i, (_, R, W, occ3, par) in enumerate(entry):
-            obj.set_variant(i, int(R), int(W), occ3=bool(occ3), par=bool(par))
+        for i, e in enumerate(entry):
+            R, W, occ3, par = e[1:5]
+            wg = e[5] if len(e) > 5 else False
+            obj.set_variant(i, int(R), int(W), occ3=bool(occ3), par=bool(par), wg=bool(wg))
         return "cache"
     (tune or obj.autotune)()
     data[key] = [[names[i], *obj.get_variant(i)] for i in range(len(names))]

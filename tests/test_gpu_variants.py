@@ -1,7 +1,7 @@
 """Every compiled kernel variant of every launch slot against the oracle.
 
-The autotuner may select any (radix, width, register budget, advection form)
-variant of a slot, so each one is checked: one slot at a time is switched to
+The autotuner may select any (radix, width, register budget, advection form,
+bricked-line lane mapping) variant of a slot, so each one is checked: one slot at a time is switched to
 the variant (the other slots keep their defaults), the problem restarts from
 the same state and takes 3 steps.  Bars: FP64 relative L2 <= 1e-12 against the
 FP64 oracle; mixed form <= 1e-5.
@@ -84,19 +84,20 @@ def test_every_variant_of_every_slot(dev, ndim, n, stepper, T):
         for R in RADICES:
             for W in WIDTHS:
                 for occ3, par in FORMS:
-                    try:
-                        p.set_variant(i, R, W, occ3=occ3, par=par)
-                    except PTFError:
-                        continue
-                    p.sol = sol0
-                    ptf.stepforward_(p, nsteps)
-                    err = rel(p.sol, o.sol)
-                    assert err <= bar, (name, R, W, occ3, par, err)
-                    count += 1
+                    for wg in (False, True):
+                        try:
+                            p.set_variant(i, R, W, occ3=occ3, par=par, wg=wg)
+                        except PTFError:
+                            continue
+                        p.sol = sol0
+                        ptf.stepforward_(p, nsteps)
+                        err = rel(p.sol, o.sol)
+                        assert err <= bar, (name, R, W, occ3, par, wg, err)
+                        count += 1
         assert count >= 1, name
         tested += count
-        R, W, occ3, par = default
-        p.set_variant(i, R, W, occ3=occ3, par=par)
+        R, W, occ3, par, wg = default
+        p.set_variant(i, R, W, occ3=occ3, par=par, wg=wg)
     print(f"  {len(names)} slots, {tested} variants", flush=True)
     with pytest.raises(PTFError):
         p.set_variant(0, 3, 8)
