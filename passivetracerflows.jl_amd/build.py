@@ -3,8 +3,8 @@
     python passivetracerflows.jl_amd/build.py [--jobs N] [--incremental]
 
 The kernel registry (csrc/ptf_registry.hip) is compiled once per
-(precision, kernel kind) so the units build in parallel; csrc/ptf_api.hip is
-the host runtime.  All HIP code is built with packed-FP32 instructions off.
+(precision, kernel kind) so the units build in parallel; csrc/ptf_member.hip holds
+the whole-member step kernels; csrc/ptf_api.hip is the host runtime.  All HIP code is built with packed-FP32 instructions off.
 """
 from __future__ import annotations
 
@@ -49,11 +49,15 @@ def _units():
                        f"-DPTF_PART={part}", f"-DPTF_NPARTS={nparts}",
                        "-c", os.path.join(CSRC, "ptf_registry.hip"), "-o", obj]
                 units.append((obj, cmd, ["ptf_registry.hip", *HEADERS]))
+    # the whole-member step kernels (small 2D ensemble members)
+    obj = os.path.join(OBJDIR, "ptf_member.o")
+    units.append((obj, [HIPCC, *COMMON, "-c", os.path.join(CSRC, "ptf_member.hip"), "-o", obj],
+                  ["ptf_member.hip", "ptf_member.hpp", *HEADERS]))
     # the heaviest units first, so that they do not end up on the critical path last
     units.sort(key=lambda u: -_weight(u[0]))
     obj = os.path.join(OBJDIR, "ptf_api.o")
     units.append((obj, [HIPCC, *COMMON, "-c", os.path.join(CSRC, "ptf_api.hip"), "-o", obj],
-                  ["ptf_api.hip", *API_HEADERS, os.path.join(INCLUDE, "ptf.h")]))
+                  ["ptf_api.hip", *API_HEADERS, "ptf_member.hpp", os.path.join(INCLUDE, "ptf.h")]))
     return units
 
 
@@ -115,7 +119,7 @@ def build_asan(verbose: bool = True) -> str:
     Load it with the clang ASan runtime preloaded (see tests/test_sanitizer.py); it is for
     the CPU tests of the boundary (argument checks, error paths, host buffers)."""
     units = _units()
-    regs = [o for o, _, _ in units if os.path.basename(o).startswith("reg_")]
+    regs = [o for o, _, _ in units if os.path.basename(o).startswith(("reg_", "ptf_member"))]
     missing = [o for o in regs if not os.path.exists(o)]
     if missing:
         raise RuntimeError("build the library first (missing " + os.path.basename(missing[0]) + ")")

@@ -6,6 +6,7 @@
 // and a per-stage list of kernel launches ("slots") whose arguments are fixed
 // at creation, so one step is a plain walk over the slot lists.
 #include "ptf.h"
+#include "ptf_member.hpp"
 #include "ptf_setup.hpp"
 
 #include <dlfcn.h>
@@ -52,6 +53,7 @@ static int fail(int code, const std::string& msg) {
 // ---------------------------------------------------------------------------
 enum {
   K_ANALYTIC = 20,  // analytic_flow_kernel (not in the variant registry)
+  K_MEMBER = 21,    // member_step_kernel (ptf_member.hip, not in the variant registry)
   K_COL_FWD = 0, K_COL_INV = 1, K_COL_D1 = 2, K_COL_D2 = 3, K_COL_STAGE = 4, K_COL_FLOWSYN = 5,
   K_COL_STAGE_D1 = 6,
   K_ROW_C2R = 10, K_ROW_R2C = 11, K_ROW_ADV1 = 12, K_ROW_ADV2 = 13, K_ROW_ADV3 = 14
@@ -126,6 +128,7 @@ static const char* kind_name(int kind) {
     case K_ROW_ADV2: return "row_advect2";
     case K_ROW_ADV3: return "row_advect3";
     case K_ANALYTIC: return "analytic_flow";
+    case K_MEMBER: return "member_step";
   }
   return "?";
 }
@@ -721,6 +724,48 @@ static int build_calcN(Group& g, const cpx<T>* in, const StageArgs<T>& st, cpx<T
   return PTF_OK;
 }
 
+// Ensemble groups of small square 2D members take the whole step in one launch, a
+// workgroup per member (member_step_kernel) where one is compiled for the size;
+// PTF_MEMBER_KERNEL=0 keeps them on the pass kernels, to measure the two.
+template <typename T>
+static bool build_member_slot(Group& g) {
+  if (g.ndim != 2 || g.nx != g.ny || g.B < 2 || g.plan_only) return false;
+  const char* e = std::getenv("PTF_MEMBER_KERNEL");
+  if (e && e[0] == '0') return false;
+  int threads = 0;
+  const void* fn = member_kernel(g.dbl, g.nx, &threads);
+  if (!fn) return false;
+  Slot s;
+  s.name = "step_member";
+  s.dbl = g.dbl;
+  s.kind = K_MEMBER;
+  s.N = g.nx;
+  s.lines = g.ny;
+  s.members = g.B;
+  s.fn = fn;
+  s.fixed_threads = threads;
+  s.fixed_blocks = (unsigned)g.B;
+  s.writes_state = true;
+  MemberArgs<T> a;
+  std::memset(&a, 0, sizeof(a));
+  a.st = stage_args<T>(g, 0);   // the kernel runs all four stages
+  a.st.store_x = 0;             // a stage input stays on chip for the next stage
+  a.st.filtered = g.filtered ? 1 : 0;   // applied by the last stage's update only
+  a.flow[0] = (const T*)g.flow[0].p;
+  a.flow[1] = (const T*)g.flow[1].p;
+  a.tw = (const cpx<T>*)g.tw[0].p;   // exp(-2 pi i m / n): the x and y tables (nx = ny)
+  a.pmstride = g.nR;
+  a.kscale_x = g.ks[0];
+  a.kscale_y = g.ks[1];
+  a.scale = 1.0 / ((double)g.nx * g.ny);
+  a.lb = g.lb;
+  set_args(s, a);
+  for (int k = 0; k < 4; ++k) g.stages[k].clear();
+  g.stages[0].push_back(s);
+  g.prologue.clear();
+  return true;
+}
+
 template <typename T>
 static int build_group_slots(Group& g) {
   // Stage-boundary fusion (2D/3D): the last kernel of a stage also performs the first inverse
@@ -729,7 +774,8 @@ static int build_group_slots(Group& g) {
   // the two against each other.
   const char* fz = std::getenv("PTF_STAGE_FUSION");
   const bool fuse = g.ndim >= 2 && !(fz && fz[0] == '0');
-  for (int s = 0; s < 4; ++s) {
+  const bool member = build_member_slot<T>(g);
+  for (int s = 0; s < 4 && !member; ++s) {
     g.stages[s].clear();
     const cpx<T>* in;
     if (g.stepper == PTF_RK4) {
@@ -743,8 +789,8 @@ static int build_group_slots(Group& g) {
     CHK(build_calcN<T>(g, in, st, nullptr, g.stages[s], "stage" + std::to_string(s + 1) + "_",
                        fuse));
   }
-  g.prologue.clear();
-  if (fuse) {
+  if (!member) g.prologue.clear();
+  if (fuse && !member) {
     const int axis = g.ndim == 3 ? 2 : 1;
     Slot s1 = make_slot(g, K_COL_D1, axis, g.B);
     ColArgs<T> a1 = col_base<T>(g, axis);

@@ -75,11 +75,19 @@ def test_ensemble_graph_equals_direct(dev):
         assert np.array_equal(a.get_sol(i), b.get_sol(i))
 
 
-def test_ensemble_member_equals_single_problem(dev):
+def test_ensemble_member_equals_single_problem(dev, monkeypatch):
+    """On the pass kernels (PTF_MEMBER_KERNEL=0) an ensemble member is bitwise the single
+    problem; the whole-member kernel (the default for groups of small members) uses other
+    radices, so it agrees to roundoff."""
     dt, nsteps = 5e-3, 10
     members = _members()[:4]
+    monkeypatch.setenv("PTF_MEMBER_KERNEL", "0")
     e = _ens(members, "fp64", dt)
     e.step(nsteps)
+    monkeypatch.delenv("PTF_MEMBER_KERNEL")
+    em = _ens(members, "fp64", dt)
+    assert any("member" in s for s in em.slots())
+    em.step(nsteps)
     TAD = ptf.TracerAdvectionDiffusion
     for i, m in enumerate(members):
         u, v = m["flow"]
@@ -88,6 +96,45 @@ def test_ensemble_member_equals_single_problem(dev):
         TAD.set_c_(p, m["c0"])
         ptf.stepforward_(p, nsteps)
         assert np.array_equal(p.sol, e.get_sol(i))
+        assert rel(em.get_sol(i), p.sol) <= 1e-13, i
+
+
+@pytest.mark.parametrize("stepper", ["RK4", "ETDRK4", "FilteredRK4"])
+def test_ensemble_member_kernels_vs_oracle(dev, stepper):
+    """Groups of 32^2, 64^2 and 128^2 members (whole-member step kernels: 32^2 and 64^2 in
+    both forms, 128^2 in the mixed form) against the oracle: FP64 <= 1e-12, mixed <= 1e-5;
+    and a time-varying flow (uploaded at each step's clock.t) on a member kernel."""
+    dt, nsteps = 4e-3, 10
+    members = []
+    for i, n in enumerate([32, 64, 32, 128, 64, 128, 64]):
+        members.append(W.ensemble_member(n, seed=2000 + i))
+    kw = dict(order=6, innerK=0.5, outerK=1.0, tol=1e-12) if stepper == "FilteredRK4" else {}
+
+    def moving(X, Y, t):
+        return (0.4 * np.cos(t) - np.sin(X) * np.cos(Y), np.cos(X) * np.sin(Y) + 0.2 * np.sin(t))
+    descs = [dict(n=m["n"], kappa=m["kappa"]) for m in members]
+    descs[1]["flow"] = moving   # one 64^2 member with a time-varying flow
+    for prec, bar in (("fp64", 1e-12), ("mixed", 1e-5)):
+        print(f"member kernels {stepper} {prec}", flush=True)
+        e = ptf.Ensemble(descs, dt=dt, precision=prec, stepper=stepper, **kw)
+        names = e.slots()
+        assert sum("member" in s for s in names) == (3 if prec == "mixed" else 2), names
+        for i, m in enumerate(members):
+            if i != 1:
+                e.set_flow(i, *m["flow"])
+            e.set_c(i, m["c0"])
+        e.step(nsteps, graph=True)
+        for i, m in enumerate(members):
+            g = O.Grid(2, m["n"])
+            if i == 1:
+                X, Y = g.gridpoints()
+                flow = (lambda t, X=X, Y=Y: moving(X, Y, t))
+            else:
+                flow = m["flow"]
+            o = O.Problem(g, flow, kappa=m["kappa"], dt=dt, stepper=stepper, **kw)
+            o.set_c(m["c0"])
+            o.stepforward(nsteps)
+            assert rel(e.get_sol(i), o.sol) <= bar, (prec, i, m["n"])
 
 
 def test_ensemble_filtered_timevarying_clock_and_diagnostics(dev):
