@@ -538,14 +538,13 @@ __device__ __forceinline__ Rk4Ops<T> rk4_load(const StageArgs<T>& a, long long u
   return o;
 }
 
-// op = the member's operator coefficients (a.op[member]), read once by the caller: a
-// read through a.op between the state stores would be repeated for every element.
+// rk4_finish with the element's linear operator L already evaluated (a caller that
+// updates the same elements in every stage computes it once)
 template <typename T, int ST = -1>
-__device__ __forceinline__ cpx<T> rk4_finish(const StageArgs<T>& a, long long ubase, uint32_t off,
-                                             const OpParams& op, int ikx, int iky, int ikz, cpx<T> N,
-                                             const Rk4Ops<T>& o, bool store = true) {
+__device__ __forceinline__ cpx<T> rk4_finish_L(const StageArgs<T>& a, long long ubase, uint32_t off,
+                                               T L, int ikx, int iky, int ikz, cpx<T> N,
+                                               const Rk4Ops<T>& o, bool store = true) {
   const int stage = ST >= 0 ? ST : a.stage;
-  const T L = (T)linear_op(a.g, op, a.nkappah, ikx, iky, ikz);
   const T dt = (T)a.dt;
   const T h2 = (T)(a.dt / 2);
   // stage 2 input x_2 = sol + dt/2 RHS_1, recomputed exactly as stage 1 formed it
@@ -605,6 +604,16 @@ __device__ __forceinline__ cpx<T> rk4_finish(const StageArgs<T>& a, long long ub
     }
     return {hr, hi_};
   }
+}
+
+// op = the member's operator coefficients (a.op[member]), read once by the caller: a
+// read through a.op between the state stores would be repeated for every element.
+template <typename T, int ST = -1>
+__device__ __forceinline__ cpx<T> rk4_finish(const StageArgs<T>& a, long long ubase, uint32_t off,
+                                             const OpParams& op, int ikx, int iky, int ikz, cpx<T> N,
+                                             const Rk4Ops<T>& o, bool store = true) {
+  return rk4_finish_L<T, ST>(a, ubase, off, (T)linear_op(a.g, op, a.nkappah, ikx, iky, ikz), ikx,
+                             iky, ikz, N, o, store);
 }
 
 // The ETDRK4 stage operands and update (coefficients are FP64 arrays; the update runs in

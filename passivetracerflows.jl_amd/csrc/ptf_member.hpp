@@ -132,6 +132,17 @@ void member_step_kernel(MemberArgs<T> a) {
   // 128 bytes of RK4 operands per lane), the RK4 operands are loaded before the transform,
   // so that their latency overlaps it; otherwise in chunks after it (one memory round trip
   // per chunk; FP64 and ETDRK4, up to 9 operands per element, in chunks of 2).
+  // the linear operator of this lane's column elements, the same in every stage (kept in
+  // registers for up to 8 elements per lane; longer columns evaluate it per stage)
+  constexpr bool kLq = RC <= 8;
+  T Lq[kLq ? RC : 1];
+  if (kLq && colt) {
+    const int tt = lane();
+    const int kx = tt / PC, jc = tt % PC;
+#pragma unroll
+    for (int q = 0; q < (kLq ? RC : 0); ++q)
+      Lq[q] = (T)linear_op(a.st.g, opm, a.st.nkappah, kx, jc + q * PC, 0);
+  }
   auto col_stage = [&](auto stc) {
     constexpr int ST = decltype(stc)::value;
     constexpr bool kPre = RC * 4 * (int)sizeof(cpx<T>) <= 128;
@@ -165,7 +176,8 @@ void member_step_kernel(MemberArgs<T> a) {
           Rk4Ops<T> oq;
           if constexpr (kPre) oq = pre[q0 + i];
           else oq = o[i];
-          x[q0 + i] = rk4_finish<T, ST>(a.st, ubase, soff(kx, ky), opm, kx, ky, 0, x[q0 + i], oq);
+          const T L = kLq ? Lq[kLq ? q0 + i : 0] : (T)linear_op(a.st.g, opm, a.st.nkappah, kx, ky, 0);
+          x[q0 + i] = rk4_finish_L<T, ST>(a.st, ubase, soff(kx, ky), L, kx, ky, 0, x[q0 + i], oq);
         }
       }
     } else {

@@ -45,6 +45,8 @@ def apply_or_tune(obj, key, path, tune=None):
         for i, e in enumerate(entry):
             R, W, occ3, par = e[1:5]
             wg = e[5] if len(e) > 5 else False
+            if int(R) == 0:
+                continue   # a fixed-shape kernel (whole-member step): nothing to select
             obj.set_variant(i, int(R), int(W), occ3=bool(occ3), par=bool(par), wg=bool(wg))
         return "cache"
     (tune or obj.autotune)()
