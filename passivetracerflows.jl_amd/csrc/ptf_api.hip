@@ -725,13 +725,16 @@ static int build_calcN(Group& g, const cpx<T>* in, const StageArgs<T>& st, cpx<T
 }
 
 // Ensemble groups of small square 2D members take the whole step in one launch, a
-// workgroup per member (member_step_kernel) where one is compiled for the size;
-// PTF_MEMBER_KERNEL=0 keeps them on the pass kernels, to measure the two.
+// workgroup per member (member_step_kernel) where one is compiled for the size, up to
+// members of kMemberMax^2.  PTF_MEMBER_KERNEL=<n> moves that limit (0: the pass kernels
+// for every group), to measure the two.
+static constexpr int kMemberMax = 128;
 template <typename T>
 static bool build_member_slot(Group& g) {
   if (g.ndim != 2 || g.nx != g.ny || g.B < 2 || g.plan_only) return false;
   const char* e = std::getenv("PTF_MEMBER_KERNEL");
-  if (e && e[0] == '0') return false;
+  const int limit = e ? std::atoi(e) : kMemberMax;
+  if (g.nx > limit) return false;
   int threads = 0;
   const void* fn = member_kernel(g.dbl, g.nx, &threads);
   if (!fn) return false;
