@@ -127,11 +127,6 @@ void member_step_kernel(MemberArgs<T> a) {
     for (int q = 0; q < RC; ++q) A[(jc + q * PC) * NXH + kx] = x[q];
   };
 
-  // Column phase of stage ST: N = FFT_y(A[:, kx]), the stage update of this lane's
-  // elements, and (ST < 3) the next stage's inverse pair.  Where registers allow (at most
-  // 128 bytes of RK4 operands per lane), the RK4 operands are loaded before the transform,
-  // so that their latency overlaps it; otherwise in chunks after it (one memory round trip
-  // per chunk; FP64 and ETDRK4, up to 9 operands per element, in chunks of 2).
   // the linear operator of this lane's column elements, the same in every stage (kept in
   // registers for up to 8 elements per lane; longer columns evaluate it per stage)
   constexpr bool kLq = RC <= 8;
@@ -143,6 +138,11 @@ void member_step_kernel(MemberArgs<T> a) {
     for (int q = 0; q < (kLq ? RC : 0); ++q)
       Lq[q] = (T)linear_op(a.st.g, opm, a.st.nkappah, kx, jc + q * PC, 0);
   }
+  // Column phase of stage ST: N = FFT_y(A[:, kx]), the stage update of this lane's
+  // elements, and (ST < 3) the next stage's inverse pair.  Where registers allow (at most
+  // 128 bytes of RK4 operands per lane), the RK4 operands are loaded before the transform,
+  // so that their latency overlaps it; otherwise in chunks after it (one memory round trip
+  // per chunk; FP64 and ETDRK4, up to 9 operands per element, in chunks of 2).
   auto col_stage = [&](auto stc) {
     constexpr int ST = decltype(stc)::value;
     constexpr bool kPre = RC * 4 * (int)sizeof(cpx<T>) <= 128;
