@@ -262,18 +262,19 @@ constexpr int line_e(int N) {
 }
 
 // LDS layouts: line c, element i
+// idx_split(c, i0, off) = idx(c, i0 + off) for a multiple of 8 `off` known at compile
+// time (a layout may fold it into an immediate offset)
 template <int NL>
 struct LayInterleaved {  // lines interleaved: consecutive lanes = consecutive lines
   __device__ __forceinline__ int idx(int c, int i) const { return i * NL + c; }
+  __device__ __forceinline__ int idx_split(int c, int i0, int off) const { return idx(c, i0 + off); }
 };
 struct LayStrided {      // runtime strides: line-major (rows contiguous) or element-major
   int sc, si;
   __device__ __forceinline__ int idx(int c, int i) const { return c * sc + i * si; }
+  __device__ __forceinline__ int idx_split(int c, int i0, int off) const { return idx(c, i0 + off); }
 };
 
-// One pass of a line FFT (see the header): the E/RP butterflies of this thread on its
-// natural-layout registers a (already read from LDS unless this is the first pass),
-// outputs stored to LDS at their Stockham positions unless this is the last pass.
 // Orders a block's LDS accesses around an exchange: a workgroup barrier, or, when every
 // line of the block lies within one wavefront (wave = true), only that wavefront's: a
 // wavefront's LDS operations complete in issue order, so the fences (which keep the
@@ -288,6 +289,9 @@ __device__ __forceinline__ void lds_sync(bool wave) {
   }
 }
 
+// One pass of a line FFT (see the header): the E/RP butterflies of this thread on its
+// natural-layout registers a (already read from LDS unless this is the first pass),
+// outputs stored to LDS at their Stockham positions unless this is the last pass.
 template <typename T, int N, int E, int DIR, int PASS, class Lay>
 __device__ __forceinline__ void fft_pass(cpx<T> (&a)[E], cpx<T>* lds, const Lay& lay, int c, int j,
                                          const cpx<T>* __restrict__ tw, int twstride, bool wave) {
@@ -301,7 +305,10 @@ __device__ __forceinline__ void fft_pass(cpx<T> (&a)[E], cpx<T>* lds, const Lay&
   if constexpr (PASS > 0) {
     lds_sync(wave);
 #pragma unroll
-    for (int q = 0; q < E; ++q) a[q] = lds[lay.idx(c, j + q * P)];
+    for (int q = 0; q < E; ++q) {
+      if constexpr (P % 8 == 0) a[q] = lds[lay.idx_split(c, j, q * P)];
+      else a[q] = lds[lay.idx(c, j + q * P)];
+    }
   }
 #pragma unroll
   for (int s = 0; s < B; ++s) {
@@ -329,7 +336,10 @@ __device__ __forceinline__ void fft_pass(cpx<T> (&a)[E], cpx<T>* lds, const Lay&
       const int b = j + s * P;
       const int base = (b / Ns) * Ns * RP + b % Ns;
 #pragma unroll
-      for (int r = 0; r < RP; ++r) lds[lay.idx(c, base + r * Ns)] = a[s + r * B];
+      for (int r = 0; r < RP; ++r) {
+        if constexpr (Ns % 8 == 0) lds[lay.idx_split(c, base, r * Ns)] = a[s + r * B];
+        else lds[lay.idx(c, base + r * Ns)] = a[s + r * B];
+      }
     }
   }
 }

@@ -102,19 +102,51 @@ __device__ __forceinline__ void xcd_remap(int& bx, int& by, int& bz) {
 template <int N, int R>
 constexpr bool col_tw_lds() { return N > R && N <= 2048; }
 
-template <typename T, int N, int R, int TX>
-constexpr int col_lds_elems() { return ((N > R) ? N * TX : 1) + (col_tw_lds<N, R>() ? N : 0); }
+// Column exchange layouts.  Passes along a middle axis (lanes that share a line element i
+// are consecutive) interleave the lines: element i of line c at i*TX + c, so those lanes
+// hit consecutive entries.  The stage kernels run along the last axis, which is bricked
+// from 64 planes on: there the lanes of one line are consecutive, and the lines are
+// stored line-major with one skew entry every 8 elements, c*S + i + i/8 with
+// S = N + N/8 + 8.  A Stockham pass writes a lane group's elements at stride Rp and reads
+// them at stride 1; the skew spreads both over distinct banks (8 lanes per 128 bytes;
+// S = 8 mod 16 puts the next line's lanes in the other half of a 256-byte group), where
+// the interleaved layout maps a group of 8 lanes to 2 to 4 bank positions.
+template <int N>
+constexpr int col_line_stride() { return N + N / 8 + 8; }
+template <int N, int TX, bool SKEW>
+struct LayCol {
+  __device__ __forceinline__ int idx(int c, int i) const {
+    if constexpr (SKEW) return c * col_line_stride<N>() + i + (i >> 3);
+    else return i * TX + c;
+  }
+  // off a multiple of 8: i0 + off skews by (i0 >> 3) + off / 8
+  __device__ __forceinline__ int idx_split(int c, int i0, int off) const {
+    if constexpr (SKEW) return c * col_line_stride<N>() + i0 + (i0 >> 3) + off + (off >> 3);
+    else return idx(c, i0 + off);
+  }
+};
+template <int MODE>
+constexpr bool col_skew() { return MODE == COL_STAGE_D1; }
+
+template <typename T, int N, int R, int TX, int MODE>
+constexpr int col_lds_line_elems() {
+  return (N > R) ? (col_skew<MODE>() ? TX * col_line_stride<N>() : N * TX) : 1;
+}
+template <typename T, int N, int R, int TX, int MODE>
+constexpr int col_lds_elems() {
+  return col_lds_line_elems<T, N, R, TX, MODE>() + (col_tw_lds<N, R>() ? N : 0);
+}
 
 template <typename T, int N, int R, int TX, int MODE, int DIR, int OCC = 0>
 __global__ __attribute__((amdgpu_flat_work_group_size(1, TX * (N / R)),
                           amdgpu_waves_per_eu(
-                              min_waves(TX * (N / R), (int)(col_lds_elems<T, N, R, TX>() * sizeof(cpx<T>)), OCC),
-                              lds_waves(TX * (N / R), (int)(col_lds_elems<T, N, R, TX>() * sizeof(cpx<T>))))))
+                              min_waves(TX * (N / R), (int)(col_lds_elems<T, N, R, TX, MODE>() * sizeof(cpx<T>)), OCC),
+                              lds_waves(TX * (N / R), (int)(col_lds_elems<T, N, R, TX, MODE>() * sizeof(cpx<T>))))))
 void col_kernel(ColArgs<T> a) {
   constexpr int P = N / R;
-  using Lay = LayInterleaved<TX>;
+  using Lay = LayCol<N, TX, col_skew<MODE>()>;
   const Lay lay{};
-  __shared__ cpx<T> lds[(N > R) ? N * TX : 1];
+  __shared__ cpx<T> lds[col_lds_line_elems<T, N, R, TX, MODE>()];
   __shared__ cpx<T> twl[col_tw_lds<N, R>() ? N : 1];
   const int t = threadIdx.x;
   const int lb = a.lb;

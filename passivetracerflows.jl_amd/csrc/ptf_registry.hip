@@ -67,7 +67,7 @@ const void* colp() {
   constexpr bool ok = R <= N && (R >= 4 || !is_pow2_len(N)) && N % R == 0 && fft_ok(N, R) &&
                       TX * P >= 16 && TX * P <= 1024 &&
                       (is_pow2_len(N) || reduced_ok(N, R, TX, 64, P, OCC)) &&
-                      (long long)col_lds_elems<T, N, R, TX>() * (long long)sizeof(cpx<T>) <= 163840;
+                      (long long)col_lds_elems<T, N, R, TX, col_mode(KIND)>() * (long long)sizeof(cpx<T>) <= 163840;
   if constexpr (ok) {
     return reinterpret_cast<const void*>(&col_kernel<T, N, R, TX, col_mode(KIND), col_dir(KIND), OCC>);
   } else {
@@ -118,7 +118,7 @@ const void* pipep() {
   constexpr int P = N / R;
   constexpr bool ok = OCC == 0 && is_pow2_len(N) && N >= 64 && N <= 1024 && (R == 4 || R == 8) && TX <= 4 &&
                       TX * P >= 32 && TX * P <= 512 &&
-                      (long long)col_lds_elems<T, N, R, TX>() * (long long)sizeof(cpx<T>) <= 65536;
+                      (long long)col_lds_elems<T, N, R, TX, COL_PLAIN>() * (long long)sizeof(cpx<T>) <= 65536;
   if constexpr (ok) {
     return reinterpret_cast<const void*>(&col_pipe_kernel<T, N, R, TX, OCC>);
   } else {
