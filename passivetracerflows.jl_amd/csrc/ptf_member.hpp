@@ -8,12 +8,14 @@
 // pass kernels exchange through memory (two spectral fields out of every inverse pass,
 // the x pass's product) stay in LDS.
 //
-// LDS: planes A and B of N rows of NXH = N/2 + 1 values, row-major (kx fastest).  Column
-// kx is read as A[ky * NXH + kx] (odd row length: a column's lanes hit distinct banks),
-// row y as A[y * NXH + k].  Column lines (ky) and row lines (half-length x transforms of
-// M = N/2 points) are held by consecutive lanes of one wavefront, and a line's own
-// entries of A or B serve as its exchange space, so the transforms order their LDS
-// accesses per wavefront; workgroup barriers separate the column and row phases.
+// LDS: planes A and B of N rows of NXH = N/2 + 1 values, row-major (kx fastest) with a
+// row stride SR >= NXH chosen so that the lanes of a column exchange (stride SR) and of a
+// row exchange (stride 1 and the Stockham strides) spread over the banks.  Column kx is
+// read as A[ky * SR + kx], row y as A[y * SR + k].  Column lines (ky) and row lines
+// (half-length x transforms of M = N/2 points) are held by consecutive lanes of one
+// wavefront, and a line's own entries of A or B serve as its exchange space, so the
+// transforms order their LDS accesses per wavefront; workgroup barriers separate the
+// column and row phases.
 //
 // Per stage (calcN! S:756-769 for a steady 2D flow, then the stage update):
 //   columns: A = IFFT_y(x), B = IFFT_y(i l x)       (stage input x; stage 1: x = sol)
@@ -44,9 +46,15 @@ constexpr int member_threads() {
   constexpr int m = nct > nrt ? nct : nrt;
   return (m + 63) / 64 * 64;
 }
+// LDS row stride of the planes (from a 64-bank model of the column and row exchanges of
+// the compiled configurations; N/2 + 1 otherwise)
+template <typename T, int N>
+constexpr int member_row_stride() {
+  return sizeof(T) == 4 ? (N == 64 ? 39 : (N == 128 ? 67 : N / 2 + 1)) : (N == 64 ? 35 : N / 2 + 1);
+}
 template <typename T, int N>
 constexpr int member_lds_bytes() {
-  return (int)((2 * N * (N / 2 + 1) + N) * sizeof(cpx<T>));
+  return (int)((2 * N * member_row_stride<T, N>() + N) * sizeof(cpx<T>));
 }
 // Register budget: two workgroups per CU where the LDS and the waves admit it (so that one
 // hides the other's latency and a group of more members than CUs runs in one round),
@@ -74,14 +82,15 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, member_threads<N, RC, R
 void member_step_kernel(MemberArgs<T> a) {
   constexpr int M = N / 2;
   constexpr int NXH = M + 1;
+  constexpr int SR = member_row_stride<T, N>();
   constexpr int PC = N / RC;      // lanes per column line
   constexpr int PR = M / RR;      // lanes per row line
   constexpr int NCT = NXH * PC;
   constexpr int NRT = N * PR;
   static_assert(64 % PC == 0 && 64 % PR == 0, "a line within one wavefront");
   static_assert(fft_ok(N, RC) && fft_ok(M, RR), "radix plans");
-  __shared__ cpx<T> A[N * NXH];
-  __shared__ cpx<T> B[N * NXH];
+  __shared__ cpx<T> A[N * SR];
+  __shared__ cpx<T> B[N * SR];
   __shared__ cpx<T> twl[N];
   const int t = threadIdx.x;
   const int member = blockIdx.x;
@@ -92,8 +101,8 @@ void member_step_kernel(MemberArgs<T> a) {
   const long long pbase = (long long)member * a.pmstride;
   const int lb = a.lb;
   const int mask = (1 << lb) - 1;
-  const LayStrided layc{1, NXH};   // column kx, element ky
-  const LayStrided layr{NXH, 1};   // row y, element k
+  const LayStrided layc{1, SR};   // column kx, element ky
+  const LayStrided layr{SR, 1};   // row y, element k
   const bool colt = t < NCT;       // column lanes: kx = t / PC, elements ky = jc + q PC
   const bool rowt = t < NRT;       // row lanes: y = t / PR, elements k = jr + q PR
   const OpParams opm = a.st.op[member];
@@ -120,11 +129,11 @@ void member_step_kernel(MemberArgs<T> a) {
     line_fft<T, N, RC, +1, LayStrided>(w, B, layc, kx, jc, twl, 1, true);
     lds_sync(true);
 #pragma unroll
-    for (int q = 0; q < RC; ++q) B[(jc + q * PC) * NXH + kx] = w[q];
+    for (int q = 0; q < RC; ++q) B[(jc + q * PC) * SR + kx] = w[q];
     line_fft<T, N, RC, +1, LayStrided>(x, A, layc, kx, jc, twl, 1, true);
     lds_sync(true);
 #pragma unroll
-    for (int q = 0; q < RC; ++q) A[(jc + q * PC) * NXH + kx] = x[q];
+    for (int q = 0; q < RC; ++q) A[(jc + q * PC) * SR + kx] = x[q];
   };
 
   // the linear operator of this lane's column elements, the same in every stage (kept in
@@ -153,7 +162,7 @@ void member_step_kernel(MemberArgs<T> a) {
     const bool rk4 = a.st.kind == EPI_RK4;
     cpx<T> x[RC];
 #pragma unroll
-    for (int q = 0; q < RC; ++q) x[q] = A[(jc + q * PC) * NXH + kx];
+    for (int q = 0; q < RC; ++q) x[q] = A[(jc + q * PC) * SR + kx];
     Rk4Ops<T> pre[kPre ? RC : 1];
     if constexpr (kPre) {
       if (rk4) {
@@ -232,8 +241,8 @@ void member_step_kernel(MemberArgs<T> a) {
 #pragma unroll
       for (int q = 0; q < RR; ++q) {
         const int k = jr + q * PR;
-        const cpx<T> Xk = times_ik(A[y * NXH + k], (T)(k * a.kscale_x));
-        const cpx<T> Xm = times_ik(A[y * NXH + (M - k)], (T)((M - k) * a.kscale_x));
+        const cpx<T> Xk = times_ik(A[y * SR + k], (T)(k * a.kscale_x));
+        const cpx<T> Xm = times_ik(A[y * SR + (M - k)], (T)((M - k) * a.kscale_x));
         z[q] = c2r_pre(Xk, Xm, twl[k], s, k == 0);
       }
       line_fft<T, M, RR, +1, LayStrided>(z, A, layr, y, jr, twl, 2, true);
@@ -251,7 +260,7 @@ void member_step_kernel(MemberArgs<T> a) {
 #pragma unroll
       for (int q = 0; q < RR; ++q) {
         const int k = jr + q * PR;
-        z[q] = c2r_pre(B[y * NXH + k], B[y * NXH + (M - k)], twl[k], s, k == 0);
+        z[q] = c2r_pre(B[y * SR + k], B[y * SR + (M - k)], twl[k], s, k == 0);
       }
       line_fft<T, M, RR, +1, LayStrided>(z, B, layr, y, jr, twl, 2, true);
 #pragma unroll
@@ -260,21 +269,21 @@ void member_step_kernel(MemberArgs<T> a) {
       line_fft<T, M, RR, -1, LayStrided>(f, A, layr, y, jr, twl, 2, true);
       lds_sync(true);
 #pragma unroll
-      for (int q = 0; q < RR; ++q) A[y * NXH + jr + q * PR] = f[q];
+      for (int q = 0; q < RR; ++q) A[y * SR + jr + q * PR] = f[q];
       lds_sync(true);
       cpx<T> X[RR];
 #pragma unroll
       for (int q = 0; q < RR; ++q) {
         const int k = jr + q * PR;
-        const cpx<T> Zm = A[y * NXH + (M - k)];   // entry M exists (k = 0 discards it)
+        const cpx<T> Zm = A[y * SR + (M - k)];   // entry M exists (k = 0 discards it)
         X[q] = k == 0 ? cpx<T>{f[q].re + f[q].im, (T)0} : r2c_post(f[q], Zm, twl[k]);
       }
       lds_sync(true);
 #pragma unroll
       for (int q = 0; q < RR; ++q) {
         const int k = jr + q * PR;
-        A[y * NXH + k] = X[q];
-        if (k == 0) A[y * NXH + M] = cpx<T>{f[q].re - f[q].im, (T)0};
+        A[y * SR + k] = X[q];
+        if (k == 0) A[y * SR + M] = cpx<T>{f[q].re - f[q].im, (T)0};
       }
     }
     __syncthreads();
