@@ -111,16 +111,43 @@ def kernel_template(variant: str) -> str:
             f"{kv.get('occ', '0')}>")
 
 
-def _counter_rows(d):
-    """(kernel name, counter, value) of every dispatch in a rocprofv3 --pmc csv output folder."""
+def _counter_rows(d, order=False):
+    """(kernel name, counter, value) of every dispatch in a rocprofv3 --pmc csv output folder;
+    with order=True (dispatch id, kernel name, counter, value), sorted by dispatch."""
     import csv
     import glob
     out = []
     for path in glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True):
         with open(path, newline="") as f:
             for r in csv.DictReader(f):
-                out.append((r["Kernel_Name"], r["Counter_Name"], float(r["Counter_Value"])))
-    return out
+                if order:
+                    out.append((int(r["Dispatch_Id"]), r["Kernel_Name"], r["Counter_Name"],
+                                float(r["Counter_Value"])))
+                else:
+                    out.append((r["Kernel_Name"], r["Counter_Name"], float(r["Counter_Value"])))
+    return sorted(out) if order else out
+
+
+def _per_launch_avg(seq, templates):
+    """Average of a family's per-dispatch values, launch position by launch position.
+
+    seq: (kernel name, value) of the dispatches of the family's kernels in dispatch order;
+    templates: the kernel of each launch of a step, in order.  The dispatches cycle through
+    the step's launches; each position is averaged over its dispatches (its own variant),
+    then the positions are averaged: the per-launch mean of the step, comparable with the
+    family's average algorithmic bytes.  None when the dispatches do not follow that cycle."""
+    L = len(templates)
+    ds = [(n, x) for (n, x) in seq if any(t in n for t in templates)]
+    if not ds or len(ds) % L:
+        ds = ds[len(ds) % L:] if ds else ds
+    if not ds:
+        return None, 0
+    pos = [[] for _ in range(L)]
+    for i, (n, x) in enumerate(ds):
+        if templates[i % L] not in n:
+            return None, 0
+        pos[i % L].append(x)
+    return sum(sum(p) / len(p) for p in pos) / L, len(ds)
 
 
 def pmc_summary(fetch_dir, write_dir, cal_fetch_dir, cal_write_dir, bench_json):
@@ -136,14 +163,17 @@ def pmc_summary(fetch_dir, write_dir, cal_fetch_dir, cal_write_dir, bench_json):
         calib[counter] = dict(kib_per_launch=kib, launches=n,
                               factor=(1 << 30) / (kib * 1024.0) if kib else None)
     bj = json.loads(open(bench_json).read().strip().splitlines()[-1])
-    rows = {c: _counter_rows(d) for c, d in (("FETCH_SIZE", fetch_dir), ("WRITE_SIZE", write_dir))}
+    rows = {c: _counter_rows(d, order=True) for c, d in (("FETCH_SIZE", fetch_dir),
+                                                          ("WRITE_SIZE", write_dir))}
     fams = {}
     for fam, k in bj["kernels"].items():
-        tname = kernel_template(k["kernel"])
-        ent = dict(kernel=tname, algorithmic_bytes_per_launch=k.get("bytes_per_launch"))
+        templates = k.get("kernel_per_launch") or [kernel_template(k["kernel"])]
+        ent = dict(kernel=k["kernel"], kernel_per_launch=templates,
+                   algorithmic_bytes_per_launch=k.get("bytes_per_launch"))
         tot = 0.0
         for counter in ("FETCH_SIZE", "WRITE_SIZE"):
-            kib, n = avg(rows[counter], counter, lambda name: tname in name)
+            seq = [(name, x) for (_, name, c, x) in rows[counter] if c == counter]
+            kib, n = _per_launch_avg(seq, templates)
             fac = calib[counter]["factor"]
             ent[counter] = dict(kib_per_launch=kib, launches=n)
             if kib is None or fac is None:
@@ -174,13 +204,15 @@ def trace_summary(trace_dir, bench_out, steps, warmup):
     lines = ["| family | kernel | launches/step | rocprof avg us (in-step dispatches) | n | "
              "bench.py HIP-event avg us | ratio |", "|---|---|---|---|---|---|---|"]
     for fam, k in bj["kernels"].items():
-        tname = kernel_template(k["kernel"])
-        d = [(e - b) for (b, e, n) in disp if tname in n]
+        templates = k.get("kernel_per_launch") or [kernel_template(k["kernel"])]
         nstep = k["launches_per_step"] * (warmup + 2 * steps)
-        use = d[-nstep:] if len(d) >= nstep else d
-        avg = sum(use) / len(use) / 1e3 if use else float("nan")
+        seq = [(n, e - b) for (b, e, n) in disp if any(t in n for t in templates)]
+        seq = seq[-nstep:] if len(seq) >= nstep else seq
+        avg, n = _per_launch_avg(seq, templates)
+        avg = avg / 1e3 if avg is not None else float("nan")
         ev = k["ms_per_launch"] * 1e3
-        lines.append(f"| {fam} | `{tname}` | {k['launches_per_step']} | {avg:.1f} | {len(use)} | "
+        shown = "`, `".join(dict.fromkeys(templates))
+        lines.append(f"| {fam} | `{shown}` | {k['launches_per_step']} | {avg:.1f} | {n} | "
                      f"{ev:.1f} | {avg / ev:.3f} |")
     return "\n".join(lines)
 
@@ -336,14 +368,27 @@ def slot_table(prob, nS, nR, cb, rb, steps, stepper="RK4"):
 
 
 def dominant(rows):
+    """Kernel families of a step (a slot name without its stage prefix): summed in-step time,
+    launches, algorithmic bytes, and the variant of every launch in step order (the stage
+    kernels are tuned per stage, so a family can mix variants); `kernel` names them."""
     fams = {}
     for r in rows:
         key = r["name"].split(" [")[0]
         key = key.split("_", 1)[1] if key.startswith("stage") else key
-        f = fams.setdefault(key, dict(total_ms=0.0, n=0, bytes=0, kernel=r["name"].split(" [")[1][:-1]))
+        f = fams.setdefault(key, dict(total_ms=0.0, n=0, bytes=0, per_launch=[]))
         f["total_ms"] += r["ms"]
         f["n"] += 1
         f["bytes"] += r["bytes"] or 0
+        f["per_launch"].append(r["name"].split(" [")[1][:-1])
+    for f in fams.values():
+        v = f["per_launch"]
+        if len(set(v)) == 1:
+            f["kernel"] = v[0]
+        else:
+            f["kernel"] = "; ".join(
+                f"{u} (launch {', '.join(str(i + 1) for i, w in enumerate(v) if w == u)})"
+                for u in dict.fromkeys(v))
+        f["templates"] = [kernel_template(u) for u in v]
     name = max(fams, key=lambda k: fams[k]["total_ms"])
     f = fams[name]
     avg_ms = f["total_ms"] / f["n"]
@@ -577,7 +622,8 @@ def profile_form(dev, form, steps, warmup, autotune):
     rows = slot_table(prob, nS, npts, cb, rb, steps, stepper)
     _, _, _, _, fams = dominant(rows)
     out.update(ms_per_step=1e3 * secs / steps, value=gps(npts, steps, secs), variants=prob.tuned,
-               kernels={k: dict(kernel=f["kernel"], ms_per_launch=f["total_ms"] / f["n"],
+               kernels={k: dict(kernel=f["kernel"], kernel_per_launch=f["templates"],
+                                ms_per_launch=f["total_ms"] / f["n"],
                                 launches_per_step=f["n"], bytes_per_launch=f["bytes"] / f["n"],
                                 GBs=(f["bytes"] / f["total_ms"] * 1e3 / 1e9) if f["bytes"] else None)
                         for k, f in fams.items()})
@@ -670,8 +716,8 @@ def main():
         with open(tpath) as f:
             tj = json.load(f)
         ent = tj.get("families", {}).get(dname, {})
-        traffic_variant = ent.get("kernel")
-        if traffic_variant == kernel_template(fam["kernel"]):
+        traffic_variant = ent.get("kernel_per_launch") or ent.get("kernel")
+        if ent.get("kernel_per_launch") == fam["templates"]:
             traffic = ent.get("hbm_bytes_per_launch")
 
     # mixed form of the same workload + its error against the FP64 run
@@ -729,11 +775,12 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "traffic_variant": traffic_variant,
                      "kernel": f"{dname} [{fam['kernel']}]",
-                     "kernel_template": kernel_template(fam["kernel"]), "variants": prob.tuned,
+                     "kernel_template": fam["templates"], "variants": prob.tuned,
                      "avg_launch_ms": avg_ms,
                      "algorithmic_bytes_per_launch": avg_bytes},
         "cpu_baseline": cpu,
-        "kernels": {k: dict(kernel=f["kernel"], ms_per_launch=f["total_ms"] / f["n"],
+        "kernels": {k: dict(kernel=f["kernel"], kernel_per_launch=f["templates"],
+                            ms_per_launch=f["total_ms"] / f["n"],
                             launches_per_step=f["n"], bytes_per_launch=f["bytes"] / f["n"],
                             GBs=(f["bytes"] / f["total_ms"] * 1e3 / 1e9) if f["bytes"] else None)
                     for k, f in fams.items()},

@@ -1311,9 +1311,13 @@ static int time_slot(Group& g, Slot& s, int reps, hipStream_t st, double* avg_ms
 // autotune: for every distinct (kind, N, lines) pick the fastest valid (R, W, form).  A
 // candidate is scored by the sum of its times over the distinct launches of that key
 // (the four stages of a stage kernel read different operands), each timed in isolation.
+// Launches that share a key get one variant.  The stage kernels of the four stages read
+// different operands, so each stage's kernel is tuned on its own (its slot name carries
+// the stage); other kernels of a step repeat per stage with the same arguments.
 static std::string tune_key(const Slot& s) {
+  const bool stage_kind = s.kind == K_COL_STAGE || s.kind == K_COL_STAGE_D1 || s.kind == K_ROW_ADV1;
   return std::to_string(s.kind) + ":" + std::to_string(s.N) + ":" + std::to_string(s.lines) +
-         ":" + std::to_string(s.members);
+         ":" + std::to_string(s.members) + (stage_kind ? ":" + s.name : std::string());
 }
 
 // In-step durations: `nsteps` whole steps of the group with an event after every launch
