@@ -86,29 +86,41 @@ def kernel_bytes(name: str, nS: int, nR: int, cb: int, rb: int, stage: int, step
     return None
 
 
-# kernel-kind name -> (template, MODE, DIR or NIN) of the HIP kernel it instantiates
+# kernel-kind name -> (template, MODE, DIR or NIN, epilogue family) of the HIP kernel it
+# instantiates; the stage kernels are compiled per epilogue family (1: RK4, 2: ETDRK4 and
+# the plain store, 3: all of them, which the FP64 RK4 kinds use), the other column kernels
+# carry family 0
 _KIND_TEMPLATE = {
-    "col_fft_fwd": ("col_kernel", 0, -1), "col_fft_inv": ("col_kernel", 0, 1),
-    "col_inv_deriv1": ("col_kernel", 1, 1), "col_inv_deriv2": ("col_kernel", 2, 1),
-    "col_fwd_stage": ("col_kernel", 3, -1), "col_flow_synth": ("col_kernel", 4, 1),
-    "col_fwd_stage_inv": ("col_kernel", 5, -1),
-    "row_c2r": ("row_kernel", 0, 1), "row_r2c": ("row_kernel", 1, 1),
-    "row_advect1": ("row_kernel", 2, 1), "row_advect2": ("row_kernel", 2, 2),
-    "row_advect3": ("row_kernel", 2, 3),
+    "col_fft_fwd": ("col_kernel", 0, -1, 0), "col_fft_inv": ("col_kernel", 0, 1, 0),
+    "col_inv_deriv1": ("col_kernel", 1, 1, 0), "col_inv_deriv2": ("col_kernel", 2, 1, 0),
+    "col_fwd_stage": ("col_kernel", 3, -1, 2), "col_flow_synth": ("col_kernel", 4, 1, 0),
+    "col_fwd_stage_inv": ("col_kernel", 5, -1, 2),
+    "col_fwd_stage_rk4": ("col_kernel", 3, -1, 1), "col_fwd_stage_inv_rk4": ("col_kernel", 5, -1, 1),
+    "row_c2r": ("row_kernel", 0, 1, None), "row_r2c": ("row_kernel", 1, 1, None),
+    "row_advect1": ("row_kernel", 2, 1, None), "row_advect2": ("row_kernel", 2, 2, None),
+    "row_advect3": ("row_kernel", 2, 3, None),
 }
 
 
 def kernel_template(variant: str) -> str:
-    """'col_fwd_stage_inv N=256 R=8 W=2 occ=3 f64' -> 'col_kernel<double, 256, 8, 2, 5, -1, 3>',
-    the kernel name the profiler reports for that variant."""
+    """'col_fwd_stage_inv_rk4 N=256 R=8 W=2 occ=3 f32' ->
+    'col_kernel<float, 256, 8, 2, 5, -1, 3, 1>', the kernel name the profiler reports for
+    that variant."""
     parts = variant.split()
     kv = dict(p.split("=") for p in parts if "=" in p)
-    tmpl, mode, last = _KIND_TEMPLATE[parts[0]]
+    tmpl, mode, last, epf = _KIND_TEMPLATE[parts[0]]
     ty = "double" if parts[-1] == "f64" else "float"
-    if "par" in parts:   # input-parallel advection: row_adv_par_kernel<T, NX, R, NL, NIN, OCC>
-        return f"row_adv_par_kernel<{ty}, {kv['N']}, {kv['R']}, {kv['W']}, {last}, {kv.get('occ', '0')}>"
-    return (f"{tmpl}<{ty}, {kv['N']}, {kv['R']}, {kv['W']}, {mode}, {last}, "
-            f"{kv.get('occ', '0')}>")
+    occ = kv.get("occ", "0")
+    if epf == 1 and ty == "double":
+        epf = 3
+    if "par" in parts:
+        if tmpl == "col_kernel":   # pipelined persistent stage kernel
+            return f"col_pipe_kernel<{ty}, {kv['N']}, {kv['R']}, {kv['W']}, {occ}, {epf}>"
+        # input-parallel advection: row_adv_par_kernel<T, NX, R, NL, NIN, OCC>
+        return f"row_adv_par_kernel<{ty}, {kv['N']}, {kv['R']}, {kv['W']}, {last}, {occ}>"
+    if tmpl == "col_kernel":
+        return f"{tmpl}<{ty}, {kv['N']}, {kv['R']}, {kv['W']}, {mode}, {last}, {occ}, {epf}>"
+    return f"{tmpl}<{ty}, {kv['N']}, {kv['R']}, {kv['W']}, {mode}, {last}, {occ}>"
 
 
 def _counter_rows(d, order=False):

@@ -29,10 +29,10 @@ COMMON = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-I", CSRC, "-I", INCLUDE, "-Wno-unused-command-line-argument",
 ]
-KINDS = [0, 1, 2, 3, 4, 5, 6, 10, 11, 12, 13, 14]
+KINDS = [0, 1, 2, 3, 4, 5, 6, 7, 8, 10, 11, 12, 13, 14]
 # parts per kind: the line lengths of the heavy kinds (stage epilogues, advection) are
 # dealt round-robin over several objects so that they compile in parallel (at most 6)
-PARTS = {3: 2, 4: 4, 5: 2, 6: 6, 12: 2, 13: 3, 14: 3}
+PARTS = {3: 2, 4: 4, 5: 2, 6: 6, 7: 4, 8: 6, 12: 2, 13: 3, 14: 3}
 PRECS = [("double", "f64"), ("float", "f32")]
 HEADERS = ["ptf_device.hpp", "ptf_kernels.hpp"]
 API_HEADERS = HEADERS + ["ptf_setup.hpp"]
@@ -66,7 +66,7 @@ def _weight(obj):
     if not name.startswith("reg_"):
         return 0
     k = int(name.split("_")[2])
-    return {6: 10, 4: 6, 12: 4, 13: 4, 14: 4, 3: 2, 5: 2}.get(k, 1) / PARTS.get(k, 1)
+    return {6: 10, 8: 10, 4: 6, 7: 6, 12: 4, 13: 4, 14: 4, 3: 2, 5: 2}.get(k, 1) / PARTS.get(k, 1)
 
 
 def _stale(obj, deps):

@@ -56,6 +56,9 @@ enum {
   K_MEMBER = 21,    // member_step_kernel (ptf_member.hip, not in the variant registry)
   K_COL_FWD = 0, K_COL_INV = 1, K_COL_D1 = 2, K_COL_D2 = 3, K_COL_STAGE = 4, K_COL_FLOWSYN = 5,
   K_COL_STAGE_D1 = 6,
+  // the stage kernels are compiled per epilogue family: kinds 4 and 6 for ETDRK4 and the
+  // plain store, 7 and 8 (the same two kernels) for RK4; stage_col_kind picks one
+  K_COL_STAGE_RK4 = 7, K_COL_STAGE_D1_RK4 = 8,
   K_ROW_C2R = 10, K_ROW_R2C = 11, K_ROW_ADV1 = 12, K_ROW_ADV2 = 13, K_ROW_ADV3 = 14
 };
 
@@ -68,7 +71,7 @@ static constexpr int kMaxParts = 6;
   PTF_DECL1(t, k, 5)
 #define PTF_DECL_ALL(t) \
   PTF_DECL(t, 0) PTF_DECL(t, 1) PTF_DECL(t, 2) PTF_DECL(t, 3) PTF_DECL(t, 4) PTF_DECL(t, 5) \
-  PTF_DECL(t, 6) \
+  PTF_DECL(t, 6) PTF_DECL(t, 7) PTF_DECL(t, 8) \
   PTF_DECL(t, 10) PTF_DECL(t, 11) PTF_DECL(t, 12) PTF_DECL(t, 13) PTF_DECL(t, 14)
 PTF_DECL_ALL(f64)
 PTF_DECL_ALL(f32)
@@ -86,7 +89,7 @@ static RegFn reg_part(bool dbl, int kind, int part) {
 #define PTF_SW(t) \
   switch (kind) { \
     PTF_P(t, 0) PTF_P(t, 1) PTF_P(t, 2) PTF_P(t, 3) PTF_P(t, 4) PTF_P(t, 5) PTF_P(t, 6) \
-    PTF_P(t, 10) PTF_P(t, 11) PTF_P(t, 12) PTF_P(t, 13) PTF_P(t, 14) \
+    PTF_P(t, 7) PTF_P(t, 8) PTF_P(t, 10) PTF_P(t, 11) PTF_P(t, 12) PTF_P(t, 13) PTF_P(t, 14) \
     default: return nullptr; \
   }
   if (part < 0 || part >= kMaxParts) return nullptr;
@@ -110,7 +113,7 @@ static const void* reg_lookup(bool dbl, int kind, int N, int R, int W) {
 }
 
 static bool kind_known(int kind) {
-  return (kind >= 0 && kind <= 6) || (kind >= 10 && kind <= 14);
+  return (kind >= 0 && kind <= 8) || (kind >= 10 && kind <= 14);
 }
 
 static const char* kind_name(int kind) {
@@ -122,6 +125,8 @@ static const char* kind_name(int kind) {
     case K_COL_STAGE: return "col_fwd_stage";
     case K_COL_FLOWSYN: return "col_flow_synth";
     case K_COL_STAGE_D1: return "col_fwd_stage_inv";
+    case K_COL_STAGE_RK4: return "col_fwd_stage_rk4";
+    case K_COL_STAGE_D1_RK4: return "col_fwd_stage_inv_rk4";
     case K_ROW_C2R: return "row_c2r";
     case K_ROW_R2C: return "row_r2c";
     case K_ROW_ADV1: return "row_advect1";
@@ -184,15 +189,15 @@ struct Slot {
   // 128-byte runs); the same test as kWaveOk in col_kernel.  The persistent pipelined
   // kernel keeps its own lane mapping.
   bool wave_ok(int r) const {
-    if (!is_col() || brick <= 0 || fixed_threads > 0 || r <= 0 || (kind == 6 && par)) return false;
+    if (!is_col() || brick <= 0 || fixed_threads > 0 || r <= 0 || ((kind == 6 || kind == 8) && par)) return false;
     const int P = N / r;
     return P <= 64 && 64 % P == 0 && (dbl || P <= 32);
   }
   int wave_lines() const { return wave_ok(R) && !wg ? 1 : 0; }
-  bool has_occ() const { return kind == 3 || kind == 6 || kind == 13 || kind == 14; }
+  bool has_occ() const { return kind == 3 || kind == 6 || kind == 8 || kind == 13 || kind == 14; }
   // par: the input-parallel advection form (kinds 13, 14) or the pipelined persistent
-  // stage-boundary kernel (kind 6)
-  bool has_par() const { return kind == 6 || kind == 13 || kind == 14; }
+  // stage-boundary kernel (kinds 6, 8)
+  bool has_par() const { return kind == 6 || kind == 8 || kind == 13 || kind == 14; }
   int threads() const {
     if (fixed_threads > 0) return fixed_threads;
     if (is_col()) return W * (N / R);
@@ -594,11 +599,18 @@ static bool writes_sol(const StageArgs<T>& st) {
   return st.stage == 3 || (st.kind == EPI_RK4 && st.stage == 2);
 }
 
+// The kernel kind of a stage's last column pass: with or without the fused first inverse
+// pass of the next stage, compiled for the epilogue family of st.kind.
+static int stage_col_kind(bool fused, int epi) {
+  if (epi == EPI_RK4) return fused ? K_COL_STAGE_D1_RK4 : K_COL_STAGE_RK4;
+  return fused ? K_COL_STAGE_D1 : K_COL_STAGE;
+}
+
 // Launch list of one calcN! evaluation of `in`, ending in `st` (a stage
 // epilogue or a plain store to `out`).
 // With fused = true the first inverse pass is left out (the previous stage's
 // last kernel produced w0/w1) and the last kernel also performs the first
-// inverse pass of the next stage's input (K_COL_STAGE_D1).
+// inverse pass of the next stage's input (K_COL_STAGE_D1 or K_COL_STAGE_D1_RK4).
 template <typename T>
 static int build_calcN(Group& g, const cpx<T>* in, const StageArgs<T>& st, cpx<T>* out,
                        std::vector<Slot>& list, const std::string& prefix, bool fused = false) {
@@ -648,7 +660,7 @@ static int build_calcN(Group& g, const cpx<T>* in, const StageArgs<T>& st, cpx<T
     CHK(pick_default(s2));
     list.push_back(s2);
 
-    Slot s3 = make_slot(g, fused ? K_COL_STAGE_D1 : K_COL_STAGE, 1, g.B);
+    Slot s3 = make_slot(g, stage_col_kind(fused, st.kind), 1, g.B);
     ColArgs<T> a3 = col_base<T>(g, 1);
     a3.in0 = w0;
     a3.out0 = fused ? w0 : out;
@@ -710,7 +722,7 @@ static int build_calcN(Group& g, const cpx<T>* in, const StageArgs<T>& st, cpx<T
   CHK(pick_default(s4));
   list.push_back(s4);
 
-  Slot s5 = make_slot(g, fused ? K_COL_STAGE_D1 : K_COL_STAGE, 2, g.B);
+  Slot s5 = make_slot(g, stage_col_kind(fused, st.kind), 2, g.B);
   ColArgs<T> a5 = col_base<T>(g, 2);
   a5.in0 = w0;
   a5.out0 = fused ? w0 : out;
@@ -1315,7 +1327,8 @@ static int time_slot(Group& g, Slot& s, int reps, hipStream_t st, double* avg_ms
 // different operands, so each stage's kernel is tuned on its own (its slot name carries
 // the stage); other kernels of a step repeat per stage with the same arguments.
 static std::string tune_key(const Slot& s) {
-  const bool stage_kind = s.kind == K_COL_STAGE || s.kind == K_COL_STAGE_D1 || s.kind == K_ROW_ADV1;
+  const bool stage_kind = s.kind == K_COL_STAGE || s.kind == K_COL_STAGE_D1 || s.kind == K_COL_STAGE_RK4 ||
+                          s.kind == K_COL_STAGE_D1_RK4 || s.kind == K_ROW_ADV1;
   return std::to_string(s.kind) + ":" + std::to_string(s.N) + ":" + std::to_string(s.lines) +
          ":" + std::to_string(s.members) + (stage_kind ? ":" + s.name : std::string());
 }

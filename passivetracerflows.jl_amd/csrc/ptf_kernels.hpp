@@ -26,6 +26,23 @@ enum {
   COL_STAGE_D1 = 5 // COL_STAGE, then COL_D1 on the next stage's input (stage boundary fusion)
 };
 
+// Epilogue families of the stage kernels (COL_STAGE, COL_STAGE_D1), each compiled on its
+// own: a kernel then holds registers for one family's operands only (the RK4 operands
+// prefetched ahead of the transform and the ETDRK4 chunks would otherwise share one
+// allocation).  The host launches a family's kernel only for its epilogue kinds.
+// EPF_ANY compiles every epilogue behind a runtime test of the kind (FP64 RK4 uses it:
+// there the RK4-only build spilled more under the 3-waves/SIMD budget).
+enum {
+  EPF_NONE = 0,    // not a stage kernel
+  EPF_RK4 = 1,     // EPI_RK4 (RK4 and filtered RK4)
+  EPF_ETD = 2,     // EPI_ETDRK4 (ETDRK4 and filtered ETDRK4) and EPI_STORE (calcN! alone)
+  EPF_ANY = 3      // any of them
+};
+template <int MODE, int EPF>
+constexpr bool epf_ok() {
+  return (MODE == COL_STAGE || MODE == COL_STAGE_D1) == (EPF != EPF_NONE);
+}
+
 template <typename T>
 struct ColArgs {
   const cpx<T>* in0;
@@ -137,12 +154,13 @@ constexpr int col_lds_elems() {
   return col_lds_line_elems<T, N, R, TX, MODE>() + (col_tw_lds<N, R>() ? N : 0);
 }
 
-template <typename T, int N, int R, int TX, int MODE, int DIR, int OCC = 0>
+template <typename T, int N, int R, int TX, int MODE, int DIR, int OCC = 0, int EPF = EPF_NONE>
 __global__ __attribute__((amdgpu_flat_work_group_size(1, TX * (N / R)),
                           amdgpu_waves_per_eu(
                               min_waves(TX * (N / R), (int)(col_lds_elems<T, N, R, TX, MODE>() * sizeof(cpx<T>)), OCC),
                               lds_waves(TX * (N / R), (int)(col_lds_elems<T, N, R, TX, MODE>() * sizeof(cpx<T>))))))
 void col_kernel(ColArgs<T> a) {
+  static_assert(epf_ok<MODE, EPF>(), "stage kernels take an epilogue family, others none");
   constexpr int P = N / R;
   using Lay = LayCol<N, TX, col_skew<MODE>()>;
   const Lay lay{};
@@ -295,7 +313,7 @@ void col_kernel(ColArgs<T> a) {
       // transform so that their latency overlaps it (FP64 radix 8/16 would spill)
       constexpr bool kPre = sizeof(T) == 4 || R <= 4;
       Rk4Ops<T> ops[kPre ? R : 1];
-      const bool rk4 = a.st.kind == EPI_RK4;
+      const bool rk4 = EPF == EPF_RK4 || (EPF == EPF_ANY && a.st.kind == EPI_RK4);
       // The stage is uniform: dispatch on it once, so that the operand loads of a chunk
       // of elements form one straight-line block and go out together (one memory round
       // trip per chunk, not one per element).  FP64 radix >= 8 (RK4) loads in two chunks;
@@ -344,7 +362,7 @@ void col_kernel(ColArgs<T> a) {
             }
           }
         });
-      } else if (a.st.kind == EPI_ETDRK4) {
+      } else if (EPF != EPF_RK4 && a.st.kind == EPI_ETDRK4) {
         stage_dispatch([&](auto stc) {
           constexpr int ST = decltype(stc)::value;
           // up to 9 operands per element: 4 elements at a time, 2 under the 3-waves/SIMD budget
@@ -431,10 +449,11 @@ void col_kernel(ColArgs<T> a) {
 template <int N, int R, int TX>
 constexpr int pipe_threads() { return TX * (N / R); }
 
-template <typename T, int N, int R, int TX, int OCC = 0>
+template <typename T, int N, int R, int TX, int OCC = 0, int EPF = EPF_RK4>
 __global__ __attribute__((amdgpu_flat_work_group_size(1, TX * (N / R)),
                           amdgpu_waves_per_eu(1, OCC > 0 ? OCC : 2)))
 void col_pipe_kernel(ColArgs<T> a) {
+  static_assert(EPF != EPF_NONE, "a stage kernel takes an epilogue family");
   constexpr int P = N / R;
   const int tiles_x = (a.ncols + TX - 1) / TX;
   const int tiles_y = a.nother;
@@ -495,7 +514,7 @@ void col_pipe_kernel(ColArgs<T> a) {
     for (int q = 0; q < R; ++q) v[q] = in0[off(u, q)];
   };
 
-  const bool rk4 = a.st.kind == EPI_RK4;
+  const bool rk4 = EPF == EPF_RK4 || (EPF == EPF_ANY && a.st.kind == EPI_RK4);
   auto stage_dispatch = [&](auto body) {
     switch (a.st.stage) {
       case 0: body(std::integral_constant<int, 0>{}); break;

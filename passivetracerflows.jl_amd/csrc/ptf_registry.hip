@@ -26,27 +26,37 @@ using namespace ptf;
 namespace {
 
 // kinds
+// (the stage kinds 4 and 6 are the ETDRK4 / plain-store family, 7 and 8 the same kernels
+// compiled for the RK4 family)
 constexpr int K_COL_FWD = 0, K_COL_INV = 1, K_COL_D1 = 2, K_COL_D2 = 3, K_COL_STAGE = 4,
-              K_COL_FLOWSYN = 5, K_COL_STAGE_D1 = 6;
+              K_COL_FLOWSYN = 5, K_COL_STAGE_D1 = 6, K_COL_STAGE_RK4 = 7, K_COL_STAGE_D1_RK4 = 8;
 constexpr int K_ROW_C2R = 10, K_ROW_R2C = 11, K_ROW_ADV1 = 12, K_ROW_ADV2 = 13, K_ROW_ADV3 = 14;
 
 constexpr int col_mode(int k) {
   return k == K_COL_FWD || k == K_COL_INV ? COL_PLAIN
          : k == K_COL_D1                  ? COL_D1
          : k == K_COL_D2                  ? COL_D2
-         : k == K_COL_STAGE               ? COL_STAGE
-         : k == K_COL_STAGE_D1            ? COL_STAGE_D1
+         : k == K_COL_STAGE || k == K_COL_STAGE_RK4       ? COL_STAGE
+         : k == K_COL_STAGE_D1 || k == K_COL_STAGE_D1_RK4 ? COL_STAGE_D1
                                           : COL_FLOWSYN;
 }
 constexpr int col_dir(int k) {
-  return (k == K_COL_FWD || k == K_COL_STAGE || k == K_COL_STAGE_D1) ? -1 : +1;
+  return (k == K_COL_FWD || col_mode(k) == COL_STAGE || col_mode(k) == COL_STAGE_D1) ? -1 : +1;
+}
+// the epilogue family a kind is compiled for (FP64 RK4: every epilogue, which spilled
+// less than the RK4-only build under the 3-waves/SIMD budget)
+template <typename T>
+constexpr int col_epf(int k) {
+  return (k == K_COL_STAGE_RK4 || k == K_COL_STAGE_D1_RK4) ? (sizeof(T) == 8 ? EPF_ANY : EPF_RK4)
+         : (k == K_COL_STAGE || k == K_COL_STAGE_D1)       ? EPF_ETD
+                                                           : EPF_NONE;
 }
 
 constexpr int row_mode(int k) { return k == K_ROW_C2R ? ROW_C2R : k == K_ROW_R2C ? ROW_R2C : ROW_ADV; }
 constexpr int row_nin(int k) { return k == K_ROW_ADV2 ? 2 : k == K_ROW_ADV3 ? 3 : 1; }
 
 // kinds that are also built with a 3-waves/SIMD register budget (OCC = 3)
-constexpr bool has_occ(int k) { return k == 3 || k == 6 || k == 13 || k == 14; }
+constexpr bool has_occ(int k) { return k == 3 || k == 6 || k == 8 || k == 13 || k == 14; }
 
 // Lengths that are not powers of two are compiled with one register layout (R =
 // line_e(N) elements per thread), the default register budget, and one width: the
@@ -69,7 +79,8 @@ const void* colp() {
                       (is_pow2_len(N) || reduced_ok(N, R, TX, 64, P, OCC)) &&
                       (long long)col_lds_elems<T, N, R, TX, col_mode(KIND)>() * (long long)sizeof(cpx<T>) <= 163840;
   if constexpr (ok) {
-    return reinterpret_cast<const void*>(&col_kernel<T, N, R, TX, col_mode(KIND), col_dir(KIND), OCC>);
+    return reinterpret_cast<const void*>(
+        &col_kernel<T, N, R, TX, col_mode(KIND), col_dir(KIND), OCC, col_epf<T>(KIND)>);
   } else {
     return nullptr;
   }
@@ -110,17 +121,17 @@ const void* rowpar() {
   }
 }
 
-// pipelined persistent stage-boundary kernel (col_pipe_kernel), kind 6 with par = 1:
+// pipelined persistent stage-boundary kernel (col_pipe_kernel), kinds 6 and 8 with par = 1:
 // power-of-two lines of 64..1024 points, radix 4 or 8, up to 4 columns per block, the
 // default register budget (it holds two tiles' data)
-template <typename T, int N, int R, int TX, int OCC>
+template <typename T, int N, int R, int TX, int OCC, int EPF>
 const void* pipep() {
   constexpr int P = N / R;
   constexpr bool ok = OCC == 0 && is_pow2_len(N) && N >= 64 && N <= 1024 && (R == 4 || R == 8) && TX <= 4 &&
                       TX * P >= 32 && TX * P <= 512 &&
                       (long long)col_lds_elems<T, N, R, TX, COL_PLAIN>() * (long long)sizeof(cpx<T>) <= 65536;
   if constexpr (ok) {
-    return reinterpret_cast<const void*>(&col_pipe_kernel<T, N, R, TX, OCC>);
+    return reinterpret_cast<const void*>(&col_pipe_kernel<T, N, R, TX, OCC, EPF>);
   } else {
     return nullptr;
   }
@@ -128,11 +139,11 @@ const void* pipep() {
 
 template <typename T, int N, int R, int KIND, int OCC>
 const void* by_w_par(int W) {
-  if constexpr (KIND == K_COL_STAGE_D1) {
+  if constexpr (KIND == K_COL_STAGE_D1 || KIND == K_COL_STAGE_D1_RK4) {
     switch (W) {
-      case 1: return pipep<T, N, R, 1, OCC>();
-      case 2: return pipep<T, N, R, 2, OCC>();
-      case 4: return pipep<T, N, R, 4, OCC>();
+      case 1: return pipep<T, N, R, 1, OCC, col_epf<T>(KIND)>();
+      case 2: return pipep<T, N, R, 2, OCC, col_epf<T>(KIND)>();
+      case 4: return pipep<T, N, R, 4, OCC, col_epf<T>(KIND)>();
       default: return nullptr;
     }
   } else if constexpr (KIND >= 10) {

@@ -24,15 +24,30 @@ def test_per_launch_average_follows_the_step_cycle():
 def test_dominant_names_each_variant_with_its_launches():
     rows = []
     for s in range(4):
-        v = "col_fwd_stage_inv N=256 R=8 W=8 occ=3 wg f64" if s != 2 else \
-            "col_fwd_stage_inv N=256 R=8 W=8 wg f64"
+        v = "col_fwd_stage_inv_rk4 N=256 R=8 W=8 occ=3 wg f64" if s != 2 else \
+            "col_fwd_stage_inv_rk4 N=256 R=8 W=8 wg f64"
         rows.append(dict(name=f"stage{s + 1}_z_forward_stage_inverse [{v}]", ms=0.2, bytes=8e8))
         rows.append(dict(name=f"stage{s + 1}_x_advect [row_advect3 N=256 R=4 W=8 occ=3 f64]",
                          ms=0.1, bytes=9e8))
     name, fam, avg_ms, avg_bytes, fams = bench.dominant(rows)
     assert name == "z_forward_stage_inverse"
-    assert fam["templates"] == ["col_kernel<double, 256, 8, 8, 5, -1, 3>"] * 2 + \
-        ["col_kernel<double, 256, 8, 8, 5, -1, 0>", "col_kernel<double, 256, 8, 8, 5, -1, 3>"]
+    assert fam["templates"] == ["col_kernel<double, 256, 8, 8, 5, -1, 3, 3>"] * 2 + \
+        ["col_kernel<double, 256, 8, 8, 5, -1, 0, 3>", "col_kernel<double, 256, 8, 8, 5, -1, 3, 3>"]
     assert "(launch 1, 2, 4)" in fam["kernel"] and "(launch 3)" in fam["kernel"]
     assert fams["x_advect"]["kernel"] == "row_advect3 N=256 R=4 W=8 occ=3 f64"
     assert abs(avg_ms - 0.2) < 1e-12 and abs(avg_bytes - 8e8) < 1e-3
+
+
+def test_kernel_template_names_every_form():
+    kt = bench.kernel_template
+    assert kt("col_fwd_stage_inv N=2048 R=8 W=1 f64") == "col_kernel<double, 2048, 8, 1, 5, -1, 0, 2>"
+    assert kt("col_fwd_stage_rk4 N=64 R=4 W=8 f32") == "col_kernel<float, 64, 4, 8, 3, -1, 0, 1>"
+    assert kt("col_fft_fwd N=256 R=4 W=8 f64") == "col_kernel<double, 256, 4, 8, 0, -1, 0, 0>"
+    assert kt("col_fwd_stage_inv_rk4 N=256 R=8 W=2 par f32") == \
+        "col_pipe_kernel<float, 256, 8, 2, 0, 1>"
+    # FP64 RK4 stage kernels are compiled with every epilogue (family 3)
+    assert kt("col_fwd_stage_inv_rk4 N=256 R=8 W=8 occ=3 wg f64") == \
+        "col_kernel<double, 256, 8, 8, 5, -1, 3, 3>"
+    assert kt("row_advect2 N=512 R=4 W=8 occ=3 f32") == "row_kernel<float, 512, 4, 8, 2, 2, 3>"
+    assert kt("row_advect3 N=256 R=4 W=8 par occ=3 f32") == \
+        "row_adv_par_kernel<float, 256, 4, 8, 3, 3>"
