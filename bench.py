@@ -303,6 +303,11 @@ def sync():
     call("ptf_device_synchronize")
 
 
+# ensemble warm-up steps: one block of 8 steps and one single step, so that both captured
+# step graphs exist before a timed region
+ENS_WARM = 9
+
+
 def time_steps(stepfn, steps):
     sync()
     t0 = time.perf_counter()
@@ -563,8 +568,10 @@ def extra_forms(dev, steps, ens_steps, autotune="cache"):
             e.set_c(i, m["c0"])
         if prec == "mixed":
             _tune(e, autotune)
-        e.step(1, graph=True)  # graph capture + first replay outside the timed region
-        k = min(ens_steps, ens_cfg["nsteps"]) - 1
+        # both step graphs (a block of 8 steps, a single step) captured and replayed once
+        # outside the timed region
+        e.step(ENS_WARM, graph=True)
+        k = min(ens_steps, ens_cfg["nsteps"]) - ENS_WARM
         s = time_steps(lambda n: e.step(n, graph=True), k)
         res[prec] = (e, s, k)
     e32, s32, k = res["mixed"]
@@ -576,11 +583,11 @@ def extra_forms(dev, steps, ens_steps, autotune="cache"):
     for i, m in enumerate(members):
         e64r.set_flow(i, *[r32(f) for f in m["flow"]])
         e64r.set_c(i, r32(m["c0"]))
-    e64r.step(k + 1, graph=True)
+    e64r.step(k + ENS_WARM, graph=True)
     out["ensemble512_fp32_graph"] = dict(value=gps(npts, k, s32), ms_per_step=1e3 * s32 / k, steps=k,
                                          points_per_step=npts)
     out["ensemble512_fp64_graph"] = dict(value=gps(npts, k, s64), ms_per_step=1e3 * s64 / k, steps=k)
-    out["ensemble512_parity"] = ensemble_parity(members, e32, e64, e64r, ens_cfg["dt"], k + 1)
+    out["ensemble512_parity"] = ensemble_parity(members, e32, e64, e64r, ens_cfg["dt"], k + ENS_WARM)
     return out
 
 
@@ -607,7 +614,7 @@ def profile_form(dev, form, steps, warmup, autotune):
             e.set_flow(i, *m["flow"])
             e.set_c(i, m["c0"])
         out["variants"] = _tune(e, autotune)
-        e.step(max(warmup, 1), graph=True)
+        e.step(max(warmup, ENS_WARM), graph=True)
         secs = time_steps(lambda n: e.step(n, graph=True), steps)
         npts = sum(m["n"] ** 2 for m in members)
         out.update(ms_per_step=1e3 * secs / steps, value=gps(npts, steps, secs),

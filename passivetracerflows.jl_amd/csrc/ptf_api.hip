@@ -1546,11 +1546,15 @@ struct ptf_ensemble {
   bool own_stream = false;
   std::vector<hipStream_t> gstreams;
   std::vector<hipEvent_t> events;
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t exec = nullptr;
+  // captured step graphs: [0] one step, [1] kEnsGraphSteps steps with each group's
+  // launches in one chain
+  hipGraph_t graph[2] = {nullptr, nullptr};
+  hipGraphExec_t exec[2] = {nullptr, nullptr};
   ~ptf_ensemble() {
-    if (exec) (void)hipGraphExecDestroy(exec);
-    if (graph) (void)hipGraphDestroy(graph);
+    for (int i = 0; i < 2; ++i) {
+      if (exec[i]) (void)hipGraphExecDestroy(exec[i]);
+      if (graph[i]) (void)hipGraphDestroy(graph[i]);
+    }
     for (auto e : events) (void)hipEventDestroy(e);
     for (auto s : gstreams) (void)hipStreamDestroy(s);
     if (own_stream && stream) (void)hipStreamDestroy(stream);
@@ -2389,10 +2393,12 @@ int ptf_ensemble_destroy(ptf_ensemble* ens) {
 }
 
 static void drop_graph(ptf_ensemble* e) {
-  if (e->exec) (void)hipGraphExecDestroy(e->exec);
-  if (e->graph) (void)hipGraphDestroy(e->graph);
-  e->exec = nullptr;
-  e->graph = nullptr;
+  for (int i = 0; i < 2; ++i) {
+    if (e->exec[i]) (void)hipGraphExecDestroy(e->exec[i]);
+    if (e->graph[i]) (void)hipGraphDestroy(e->graph[i]);
+    e->exec[i] = nullptr;
+    e->graph[i] = nullptr;
+  }
 }
 
 int ptf_ensemble_set_stream(ptf_ensemble* ens, void* s) {
@@ -2461,17 +2467,44 @@ int ptf_ensemble_get_sol(ptf_ensemble* ens, int32_t member, double* sol) {
   return get_member_sol(*g, i, sol, ens->stream);
 }
 
-static int ensemble_record_step(ptf_ensemble* e) {
+// nsteps steps of every group: each group's launches go to its own stream, forked from
+// and joined to the ensemble's stream.  The groups share no buffers, so with nsteps > 1 a
+// group with less work per step runs ahead instead of waiting for the slowest group at
+// every step.
+static int ensemble_record_steps(ptf_ensemble* e, int nsteps) {
   hipStream_t s0 = e->stream;
   const int ng = (int)e->groups.size();
   hipEvent_t fork = e->events[ng];
   HIPCHK(hipEventRecord(fork, s0));
   for (int k = 0; k < ng; ++k) {
     HIPCHK(hipStreamWaitEvent(e->gstreams[k], fork, 0));
-    CHK(group_step_launches(*e->groups[k], e->gstreams[k], 0.0));
+    for (int i = 0; i < nsteps; ++i) CHK(group_step_launches(*e->groups[k], e->gstreams[k], 0.0));
     HIPCHK(hipEventRecord(e->events[k], e->gstreams[k]));
   }
   for (int k = 0; k < ng; ++k) HIPCHK(hipStreamWaitEvent(s0, e->events[k], 0));
+  return PTF_OK;
+}
+
+// Steps per replay of the multi-step ensemble graph (the groups join once per block).
+static constexpr int kEnsGraphSteps = 8;
+
+// capture graph `which` (0: one step, 1: kEnsGraphSteps steps) if it is not there yet
+static int ensemble_graph(ptf_ensemble* ens, int which) {
+  if (ens->exec[which]) return PTF_OK;
+  HIPCHK(hipStreamBeginCapture(ens->stream, hipStreamCaptureModeThreadLocal));
+  int rc = ensemble_record_steps(ens, which ? kEnsGraphSteps : 1);
+  hipGraph_t graph = nullptr;
+  hipError_t ce = hipStreamEndCapture(ens->stream, &graph);
+  if (rc != PTF_OK) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return rc;
+  }
+  HIPCHK(ce);
+  ens->graph[which] = graph;
+  if (hipGraphInstantiate(&ens->exec[which], graph, nullptr, nullptr, 0) != hipSuccess) {
+    ens->exec[which] = nullptr;
+    return fail(PTF_ERR_DEVICE, "hipGraphInstantiate failed");
+  }
   return PTF_OK;
 }
 
@@ -2519,24 +2552,16 @@ int ptf_ensemble_diagnostics(ptf_ensemble* ens, int32_t member, double* mass, do
 static int ensemble_step_impl(ptf_ensemble* ens, int64_t nsteps, int32_t use_graph) {
   for (auto& g : ens->groups) CHK(group_prime(*g, ens->stream));
   if (!use_graph) {
-    for (int64_t i = 0; i < nsteps; ++i) CHK(ensemble_record_step(ens));
+    for (int64_t i = 0; i < nsteps; ++i) CHK(ensemble_record_steps(ens, 1));
     HIPCHK(hipGetLastError());
     return PTF_OK;
   }
-  if (!ens->exec) {
-    HIPCHK(hipStreamBeginCapture(ens->stream, hipStreamCaptureModeThreadLocal));
-    int rc = ensemble_record_step(ens);
-    hipGraph_t graph = nullptr;
-    hipError_t ce = hipStreamEndCapture(ens->stream, &graph);
-    if (rc != PTF_OK) {
-      if (graph) (void)hipGraphDestroy(graph);
-      return rc;
-    }
-    HIPCHK(ce);
-    ens->graph = graph;
-    HIPCHK(hipGraphInstantiate(&ens->exec, ens->graph, nullptr, nullptr, 0));
-  }
-  for (int64_t i = 0; i < nsteps; ++i) HIPCHK(hipGraphLaunch(ens->exec, ens->stream));
+  // blocks of kEnsGraphSteps steps, then single steps
+  const int64_t blocks = nsteps / kEnsGraphSteps, rest = nsteps % kEnsGraphSteps;
+  if (blocks > 0) CHK(ensemble_graph(ens, 1));
+  if (rest > 0) CHK(ensemble_graph(ens, 0));
+  for (int64_t i = 0; i < blocks; ++i) HIPCHK(hipGraphLaunch(ens->exec[1], ens->stream));
+  for (int64_t i = 0; i < rest; ++i) HIPCHK(hipGraphLaunch(ens->exec[0], ens->stream));
   return PTF_OK;
 }
 

@@ -425,6 +425,19 @@ __device__ __forceinline__ cpx<T> ld_nt(const cpx<T>* p) {
   return {__builtin_nontemporal_load(&p->re), __builtin_nontemporal_load(&p->im)};
 }
 __device__ __forceinline__ double ld_nt(const double* p) { return __builtin_nontemporal_load(p); }
+// A load that is non-temporal only where NT.  A block that covers part of each 128-byte
+// line (the sibling block, on the same XCD, reads the rest) loads through the cache, so
+// that the sibling finds the line in L2 instead of fetching it from memory again.
+template <bool NT, typename T>
+__device__ __forceinline__ cpx<T> ld_pol(const cpx<T>* p) {
+  if constexpr (NT) return ld_nt(p);
+  else return *p;
+}
+template <bool NT>
+__device__ __forceinline__ double ld_pol(const double* p) {
+  if constexpr (NT) return ld_nt(p);
+  else return *p;
+}
 template <typename T>
 __device__ __forceinline__ void st_nt(cpx<T>* p, cpx<T> v) {
   __builtin_nontemporal_store(v.re, &p->re);
@@ -525,25 +538,26 @@ struct Rk4Ops {
 // The RK4 stage operands and update.  ST >= 0 fixes the stage at compile time (callers
 // that dispatch on the uniform stage once, so that the loads of several elements form
 // one straight-line block); ST = -1 reads it from the arguments.  store = false (lanes
-// of a tail column that only computes) suppresses the state stores.
-template <typename T, int ST = -1>
+// of a tail column that only computes) suppresses the state stores.  NTS: the state
+// loads are non-temporal (the caller's blocks cover whole 128-byte lines of the state).
+template <typename T, int ST = -1, bool NTS = true>
 __device__ __forceinline__ Rk4Ops<T> rk4_load(const StageArgs<T>& a, long long ubase, uint32_t off) {
   const int stage = ST >= 0 ? ST : a.stage;
   // every field is defined on every path so that the value stays in registers
   const cpx<T> z = {(T)0, (T)0};
   Rk4Ops<T> o = {z, z, z, z};
   if (stage == 0) {
-    o.x = ld_nt(a.sol + ubase + off);
+    o.x = ld_pol<NTS>(a.sol + ubase + off);
   } else if (stage == 1) {
-    o.s = ld_nt(a.sol + ubase + off);
-    o.ac = ld_nt(a.acc + ubase + off);
+    o.s = ld_pol<NTS>(a.sol + ubase + off);
+    o.ac = ld_pol<NTS>(a.acc + ubase + off);
   } else {
-    o.x = ld_nt(a.stg + ubase + off);
-    o.s = ld_nt(a.sol + ubase + off);
+    o.x = ld_pol<NTS>(a.stg + ubase + off);
+    o.s = ld_pol<NTS>(a.sol + ubase + off);
     // the compensated low part exists only in the mixed (FP32) form
     constexpr bool kMixed = sizeof(T) == 4;
-    if (stage == 2 || (kMixed && a.sol_lo)) o.ac = ld_nt(a.acc + ubase + off);
-    if (kMixed && stage == 3 && a.sol_lo) o.lo = ld_nt(a.sol_lo + ubase + off);
+    if (stage == 2 || (kMixed && a.sol_lo)) o.ac = ld_pol<NTS>(a.acc + ubase + off);
+    if (kMixed && stage == 3 && a.sol_lo) o.lo = ld_pol<NTS>(a.sol_lo + ubase + off);
   }
   return o;
 }
@@ -635,7 +649,9 @@ struct EtdOps {
   double e2, z, e, al, be, g;
 };
 
-template <typename T, int ST = -1>
+// NTS, NTC: the loads of the state and of the (FP64 real) coefficients are non-temporal
+// (as for rk4_load: where the caller's blocks cover whole 128-byte lines of them)
+template <typename T, int ST = -1, bool NTS = true, bool NTC = true>
 __device__ __forceinline__ EtdOps<T> etd_load(const StageArgs<T>& a, long long ubase, uint32_t off) {
   const int stage = ST >= 0 ? ST : a.stage;
   const cpx<T> zc = {(T)0, (T)0};
@@ -643,34 +659,34 @@ __device__ __forceinline__ EtdOps<T> etd_load(const StageArgs<T>& a, long long u
   // the compensated low parts exist only in the mixed (FP32) form
   constexpr bool kMixed = sizeof(T) == 4;
   if (stage <= 2) {
-    const cpx<T> s0 = ld_nt(a.sol + ubase + off);
+    const cpx<T> s0 = ld_pol<NTS>(a.sol + ubase + off);
     o.sr = (double)s0.re;
     o.si = (double)s0.im;
     if (kMixed && a.sol_lo) {
-      const cpx<T> lo = ld_nt(a.sol_lo + ubase + off);
+      const cpx<T> lo = ld_pol<NTS>(a.sol_lo + ubase + off);
       o.sr += (double)lo.re;
       o.si += (double)lo.im;
     }
-    o.e2 = ld_nt(a.E2 + ubase + off);
-    o.z = ld_nt(a.zeta + ubase + off);
+    o.e2 = ld_pol<NTC>(a.E2 + ubase + off);
+    o.z = ld_pol<NTC>(a.zeta + ubase + off);
     if (stage == 2) {
-      o.s1 = ld_nt(a.stg + ubase + off);
-      o.n1 = ld_nt(a.n1 + ubase + off);
-      o.n2 = ld_nt(a.n2 + ubase + off);
-      o.e = ld_nt(a.E + ubase + off);
-      o.al = ld_nt(a.alpha + ubase + off);
-      o.be = ld_nt(a.beta + ubase + off);
+      o.s1 = ld_pol<NTS>(a.stg + ubase + off);
+      o.n1 = ld_pol<NTS>(a.n1 + ubase + off);
+      o.n2 = ld_pol<NTS>(a.n2 + ubase + off);
+      o.e = ld_pol<NTC>(a.E + ubase + off);
+      o.al = ld_pol<NTC>(a.alpha + ubase + off);
+      o.be = ld_pol<NTC>(a.beta + ubase + off);
     }
   } else {
-    const cpx<T> ac = ld_nt(a.acc + ubase + off);
+    const cpx<T> ac = ld_pol<NTS>(a.acc + ubase + off);
     o.sr = (double)ac.re;
     o.si = (double)ac.im;
     if (kMixed && a.sol_lo) {
-      const cpx<T> al = ld_nt(a.n2 + ubase + off);   // the low part of acc
+      const cpx<T> al = ld_pol<NTS>(a.n2 + ubase + off);   // the low part of acc
       o.sr += (double)al.re;
       o.si += (double)al.im;
     }
-    o.g = ld_nt(a.gam + ubase + off);
+    o.g = ld_pol<NTC>(a.gam + ubase + off);
   }
   return o;
 }

@@ -162,6 +162,12 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, TX * (N / R)),
 void col_kernel(ColArgs<T> a) {
   static_assert(epf_ok<MODE, EPF>(), "stage kernels take an epilogue family, others none");
   constexpr int P = N / R;
+  // Non-temporal loads only where a block covers whole 128-byte lines of a field: along
+  // the bricked axis a column's run is the 8 planes of a brick, so a block reads TX * 8
+  // consecutive elements; the columns of the 3D y pass are contiguous (TX elements).
+  constexpr int kRun = MODE == COL_D2 ? TX : TX * 8;
+  constexpr bool kNtS = kRun * (int)sizeof(cpx<T>) >= 128;   // spectral fields, state
+  constexpr bool kNtC = kRun * (int)sizeof(double) >= 128;   // FP64 real coefficients
   using Lay = LayCol<N, TX, col_skew<MODE>()>;
   const Lay lay{};
   __shared__ cpx<T> lds[col_lds_line_elems<T, N, R, TX, MODE>()];
@@ -213,24 +219,27 @@ void col_kernel(ColArgs<T> a) {
   if constexpr (MODE == COL_FLOWSYN) {
     const cpx<double>* ps = a.psih0 + ubase;
     const double* om = a.omega + ubase;
-    // out1 = IFFT_y(i kr psi(t)) first, then out0 = IFFT_y(-i l psi(t))
+    // out1 = IFFT_y(i kr psi(t)) first, then out0 = IFFT_y(-i l psi(t)); psi(t) is formed
+    // once (one read of psih0 and omega), both inputs at once
+    cpx<T> w[R];
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+      const int iy = j + q * P;
+      const cpx<double> p0 = ps[off(q)];
+      const double ph = -om[off(q)] * a.t;
+      double sn, cs;
+      sincos(ph, &sn, &cs);
+      const cpx<double> p = {p0.re * cs - p0.im * sn, p0.re * sn + p0.im * cs};
+      const double kr = ikx_col * a.kscale_x;
+      v[q] = {(T)(-(kr * p.im)), (T)(kr * p.re)};          // i kr p
+      const double l = fftfreq_index(iy, N) * a.kscale;
+      w[q] = {(T)(l * p.im), (T)(-(l * p.re))};             // -i l p
+    }
 #pragma unroll
     for (int pass = 0; pass < 2; ++pass) {
+      if (pass == 1) {
 #pragma unroll
-      for (int q = 0; q < R; ++q) {
-        const int iy = j + q * P;
-        const cpx<double> p0 = ps[off(q)];
-        const double ph = -om[off(q)] * a.t;
-        double sn, cs;
-        sincos(ph, &sn, &cs);
-        const cpx<double> p = {p0.re * cs - p0.im * sn, p0.re * sn + p0.im * cs};
-        if (pass == 0) {
-          const double kr = ikx_col * a.kscale_x;
-          v[q] = {(T)(-(kr * p.im)), (T)(kr * p.re)};          // i kr p
-        } else {
-          const double l = fftfreq_index(iy, N) * a.kscale;
-          v[q] = {(T)(l * p.im), (T)(-(l * p.re))};             // -i l p
-        }
+        for (int q = 0; q < R; ++q) v[q] = w[q];
       }
       fft(std::integral_constant<int, 1>{}, v);
       cpx<T>* o = (pass == 0 ? a.out1 : a.out0) + ubase;
@@ -245,7 +254,7 @@ void col_kernel(ColArgs<T> a) {
 #pragma unroll
     for (int q = 0; q < R; ++q) {
       // the 3D y-inverse pass reads its inputs once: non-temporal loads there
-      if constexpr (MODE == COL_D2) v[q] = ld_nt(in0 + off(q));
+      if constexpr (MODE == COL_D2) v[q] = ld_pol<kNtS>(in0 + off(q));
       else v[q] = in0[off(q)];
     }
 
@@ -269,7 +278,7 @@ void col_kernel(ColArgs<T> a) {
       if constexpr (kKeep && MODE == COL_D2) {
         const cpx<T>* in1 = a.in1 + ubase;
 #pragma unroll
-        for (int q = 0; q < R; ++q) p2[q] = ld_nt(in1 + off(q));
+        for (int q = 0; q < R; ++q) p2[q] = ld_pol<kNtS>(in1 + off(q));
       }
 #pragma unroll
       for (int q = 0; q < R; ++q) {
@@ -299,7 +308,7 @@ void col_kernel(ColArgs<T> a) {
 #pragma unroll
         for (int q = 0; q < R; ++q) {
           if constexpr (kKeep) v[q] = p2[q];
-          else v[q] = ld_nt(in1 + off(q));
+          else v[q] = ld_pol<kNtS>(in1 + off(q));
         }
         fft(std::integral_constant<int, 1>{}, v);
         cpx<T>* o = a.out2 + ubase;
@@ -331,7 +340,7 @@ void col_kernel(ColArgs<T> a) {
           stage_dispatch([&](auto stc) {
             constexpr int ST = decltype(stc)::value;
 #pragma unroll
-            for (int q = 0; q < R; ++q) ops[q] = rk4_load<T, ST>(a.st, ubase, off(q));
+            for (int q = 0; q < R; ++q) ops[q] = rk4_load<T, ST, kNtS>(a.st, ubase, off(q));
           });
         }
       }
@@ -347,7 +356,7 @@ void col_kernel(ColArgs<T> a) {
             Rk4Ops<T> o[kChunk];
             if constexpr (!kPre) {
 #pragma unroll
-              for (int i = 0; i < kChunk; ++i) o[i] = rk4_load<T, ST>(a.st, ubase, off(q0 + i));
+              for (int i = 0; i < kChunk; ++i) o[i] = rk4_load<T, ST, kNtS>(a.st, ubase, off(q0 + i));
             }
 #pragma unroll
             for (int i = 0; i < kChunk; ++i) {
@@ -371,7 +380,7 @@ void col_kernel(ColArgs<T> a) {
           for (int q0 = 0; q0 < R; q0 += kChunk) {
             EtdOps<T> o[kChunk];
 #pragma unroll
-            for (int i = 0; i < kChunk; ++i) o[i] = etd_load<T, ST>(a.st, ubase, off(q0 + i));
+            for (int i = 0; i < kChunk; ++i) o[i] = etd_load<T, ST, kNtS, kNtC>(a.st, ubase, off(q0 + i));
 #pragma unroll
             for (int i = 0; i < kChunk; ++i) {
               const int q = q0 + i;
@@ -455,6 +464,9 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, TX * (N / R)),
 void col_pipe_kernel(ColArgs<T> a) {
   static_assert(EPF != EPF_NONE, "a stage kernel takes an epilogue family");
   constexpr int P = N / R;
+  // non-temporal operand loads where a tile covers whole 128-byte lines (as in col_kernel)
+  constexpr bool kNtS = TX * 8 * (int)sizeof(cpx<T>) >= 128;
+  constexpr bool kNtC = TX * 8 * (int)sizeof(double) >= 128;
   const int tiles_x = (a.ncols + TX - 1) / TX;
   const int tiles_y = a.nother;
   const int tiles_total = tiles_x * tiles_y * a.nmembers;
@@ -560,7 +572,7 @@ void col_pipe_kernel(ColArgs<T> a) {
       Rk4Ops<T> ops[R];
       load_line(cur, v);
 #pragma unroll
-      for (int q = 0; q < R; ++q) ops[q] = rk4_load<T, ST>(a.st, cur.ubase, off(cur, q));
+      for (int q = 0; q < R; ++q) ops[q] = rk4_load<T, ST, kNtS>(a.st, cur.ubase, off(cur, q));
       for (;;) {
         const unsigned Ln = L + per_xcd_blocks;
         const bool more = Ln < hi;
@@ -572,7 +584,7 @@ void col_pipe_kernel(ColArgs<T> a) {
           nxt = tile_at(Ln);
           load_line(nxt, vn);
 #pragma unroll
-          for (int q = 0; q < R; ++q) opsn[q] = rk4_load<T, ST>(a.st, nxt.ubase, off(nxt, q));
+          for (int q = 0; q < R; ++q) opsn[q] = rk4_load<T, ST, kNtS>(a.st, nxt.ubase, off(nxt, q));
         }
         fft(std::integral_constant<int, -1>{}, v);
         const OpParams opm = a.st.op[cur.member];
@@ -622,7 +634,7 @@ void col_pipe_kernel(ColArgs<T> a) {
         for (int q0 = 0; q0 < R; q0 += kChunk) {
           EtdOps<T> o[kChunk];
 #pragma unroll
-          for (int i = 0; i < kChunk; ++i) o[i] = etd_load<T, ST>(a.st, cur.ubase, off(cur, q0 + i));
+          for (int i = 0; i < kChunk; ++i) o[i] = etd_load<T, ST, kNtS, kNtC>(a.st, cur.ubase, off(cur, q0 + i));
 #pragma unroll
           for (int i = 0; i < kChunk; ++i) {
             const int q = q0 + i;
@@ -721,26 +733,35 @@ __device__ __forceinline__ void stage_row_pairs(const cpx<T> (&x)[R], cpx<T> las
 }
 
 // Row element k is row[k << lb] (brick layout; lb = 0 for contiguous rows).  The x
-// pass reads each spectral row once per stage: non-temporal loads.
-template <typename T, int M, int R>
+// pass reads each spectral row once per stage: non-temporal loads (NT), where the
+// block holds whole bricks of rows (see row_nt).
+template <typename T, int M, int R, bool NT = true>
 __device__ __forceinline__ void load_row(const cpx<T>* __restrict__ row, cpx<T> (&x)[R],
                                          cpx<T>& last, int j, int lb) {
   constexpr int P = M / R;
 #pragma unroll
-  for (int q = 0; q < R; ++q) x[q] = ld_nt(row + ((uint32_t)(j + q * P) << lb));
-  if (j == 0) last = ld_nt(row + ((uint32_t)M << lb));
+  for (int q = 0; q < R; ++q) x[q] = ld_pol<NT>(row + ((uint32_t)(j + q * P) << lb));
+  if (j == 0) last = ld_pol<NT>(row + ((uint32_t)M << lb));
 }
 
-template <typename T, int M, int R, class Lay>
+template <typename T, int M, int R, class Lay, bool NT = true>
 __device__ __forceinline__ void load_row_pairs(const cpx<T>* __restrict__ row, cpx<T> (&x)[R],
                                                cpx<T> (&xm)[R], cpx<T>* lds, const Lay& lay, int c,
                                                int j, int lb) {
   cpx<T> last = {(T)0, (T)0};
-  load_row<T, M, R>(row, x, last, j, lb);
+  load_row<T, M, R, NT>(row, x, last, j, lb);
   stage_row_pairs<T, M, R, Lay>(x, last, xm, lds, lay, c, j);
 }
 
 constexpr int ilog2(int n) { return n <= 1 ? 0 : 1 + ilog2(n / 2); }
+
+// A block of NL rows reads the spectral rows and flows of whole 8-plane bricks (every
+// 128-byte line it touches entirely) when NL >= 8: non-temporal loads.  A block with part
+// of a brick shares each line with its sibling blocks (renumbered onto one XCD), which
+// find it in L2 when it was loaded through the cache.  (Contiguous rows, lb = 0, are
+// whole lines either way; there the choice only affects what stays in the cache.)
+template <int NL>
+constexpr bool row_nt() { return NL >= 8; }
 
 // Row r of a bricked array starts at ((r >> lb) * len << lb) + (r mod 2^lb), len = the
 // row length in elements (nx/2 + 1 spectral values, or nx/2 physical x-pairs).  A block
@@ -812,7 +833,7 @@ void row_kernel(RowArgs<T> a) {
   if constexpr (MODE == ROW_C2R) {
     cpx<T> z[R], zm[R];
     const T s = (T)a.scale;
-    load_row_pairs<T, M, R, Lay>(a.in[0] + sub + so, z, zm, lds, lay, c, j, lb);
+    load_row_pairs<T, M, R, Lay, row_nt<NL>()>(a.in[0] + sub + so, z, zm, lds, lay, c, j, lb);
 #pragma unroll
     for (int q = 0; q < R; ++q) {
       const int k = j + q * P;
@@ -844,7 +865,7 @@ void row_kernel(RowArgs<T> a) {
       constexpr bool kPre = sizeof(T) == 4 || R <= 4;
       cpx<T> nxt[kPre ? R : 1];
       cpx<T> nlast = zero;
-      if constexpr (kPre) load_row<T, M, R>(a.in[0] + sub + so, nxt, nlast, j, lb);
+      if constexpr (kPre) load_row<T, M, R, row_nt<NL>()>(a.in[0] + sub + so, nxt, nlast, j, lb);
 #pragma unroll 1
       for (int d = 0; d < NIN; ++d) {
         cpx<T> z[R], zm[R];
@@ -852,15 +873,15 @@ void row_kernel(RowArgs<T> a) {
 #pragma unroll
           for (int q = 0; q < R; ++q) z[q] = nxt[q];
           stage_row_pairs<T, M, R, Lay>(z, nlast, zm, lds, lay, c, j);
-          if (d + 1 < NIN) load_row<T, M, R>(in_at(d + 1) + sub + so, nxt, nlast, j, lb);
+          if (d + 1 < NIN) load_row<T, M, R, row_nt<NL>()>(in_at(d + 1) + sub + so, nxt, nlast, j, lb);
         } else {
-          load_row_pairs<T, M, R, Lay>(in_at(d) + sub + so, z, zm, lds, lay, c, j, lb);
+          load_row_pairs<T, M, R, Lay, row_nt<NL>()>(in_at(d) + sub + so, z, zm, lds, lay, c, j, lb);
         }
         const cpx<T>* fl = reinterpret_cast<const cpx<T>*>(flow_at(d) + pub);
         cpx<T> fv[kPre ? R : 1];
         if constexpr (kPre) {
 #pragma unroll
-          for (int q = 0; q < R; ++q) fv[q] = ld_nt(fl + po + ((uint32_t)(j + q * P) << lb));
+          for (int q = 0; q < R; ++q) fv[q] = ld_pol<row_nt<NL>()>(fl + po + ((uint32_t)(j + q * P) << lb));
         }
 #pragma unroll
         for (int q = 0; q < R; ++q) {
@@ -879,7 +900,7 @@ void row_kernel(RowArgs<T> a) {
         for (int q = 0; q < R; ++q) {
           cpx<T> uu;
           if constexpr (kPre) uu = fv[q];
-          else uu = ld_nt(fl + po + ((uint32_t)(j + q * P) << lb));
+          else uu = ld_pol<row_nt<NL>()>(fl + po + ((uint32_t)(j + q * P) << lb));
           if (d == 0) {
             f[q] = {-uu.re * z[q].re, -uu.im * z[q].im};
           } else {
@@ -1000,11 +1021,11 @@ void row_adv_par_kernel(RowArgs<T> a) {
   const cpx<T>* in = grp == 0 ? a.in[0] : (grp == 1 ? a.in[1] : a.in[2]);
   const T* flp = grp == 0 ? a.flow[0] : (grp == 1 ? a.flow[1] : a.flow[2]);
   cpx<T> z[R], last = {(T)0, (T)0};
-  load_row<T, M, R>(in + sub + so, z, last, j, lb);
+  load_row<T, M, R, row_nt<NL>()>(in + sub + so, z, last, j, lb);
   const cpx<T>* fl = reinterpret_cast<const cpx<T>*>(flp + pub);
   cpx<T> fv[R];
 #pragma unroll
-  for (int q = 0; q < R; ++q) fv[q] = ld_nt(fl + po + ((uint32_t)(j + q * P) << lb));
+  for (int q = 0; q < R; ++q) fv[q] = ld_pol<row_nt<NL>()>(fl + po + ((uint32_t)(j + q * P) << lb));
   cpx<T> zm[R];
   stage_row_pairs<T, M, R, Lay>(z, last, zm, lds, lay, c, j);
 #pragma unroll

@@ -60,19 +60,18 @@ def test_ensemble_fp64_and_mixed_vs_oracle(dev):
 
 
 def test_ensemble_graph_equals_direct(dev):
+    """Graph replay (blocks of 8 steps with each size group in its own chain, then single
+    steps) is bitwise the direct launches, across several calls with host work between."""
     dt = 5e-3
-    members = _members()[:5]
+    members = _members()[:5]       # groups of 64^2, 128^2 and 256^2 members
     a = _ens(members, "fp64", dt)
     b = _ens(members, "fp64", dt)
-    a.step(7, graph=True)
-    b.step(7, graph=False)
-    for i in range(len(members)):
-        assert np.array_equal(a.get_sol(i), b.get_sol(i))
-    # the graph replays across several calls, with host work in between
-    a.step(3, graph=True)
-    b.step(3, graph=False)
-    for i in range(len(members)):
-        assert np.array_equal(a.get_sol(i), b.get_sol(i))
+    for n in (11, 9, 3, 16):       # 1 block + 3, 1 block + 1, 3 single steps, 2 blocks
+        a.step(n, graph=True)
+        b.step(n, graph=False)
+        for i in range(len(members)):
+            assert np.array_equal(a.get_sol(i), b.get_sol(i)), (n, i)
+        assert a.clock(0) == b.clock(0)
 
 
 def test_ensemble_member_equals_single_problem(dev, monkeypatch):
