@@ -443,6 +443,13 @@ __device__ __forceinline__ void st_nt(cpx<T>* p, cpx<T> v) {
   __builtin_nontemporal_store(v.re, &p->re);
   __builtin_nontemporal_store(v.im, &p->im);
 }
+// A store that is non-temporal only where NT (as ld_pol: a block that writes part of each
+// 128-byte line stores through the cache, where the sibling block's half joins it).
+template <bool NT, typename T>
+__device__ __forceinline__ void st_pol(cpx<T>* p, cpx<T> v) {
+  if constexpr (NT) st_nt(p, v);
+  else *p = v;
+}
 
 template <typename T>
 struct StageArgs {
@@ -564,7 +571,7 @@ __device__ __forceinline__ Rk4Ops<T> rk4_load(const StageArgs<T>& a, long long u
 
 // rk4_finish with the element's linear operator L already evaluated (a caller that
 // updates the same elements in every stage computes it once)
-template <typename T, int ST = -1>
+template <typename T, int ST = -1, bool NTS = true>
 __device__ __forceinline__ cpx<T> rk4_finish_L(const StageArgs<T>& a, long long ubase, uint32_t off,
                                                T L, int ikx, int iky, int ikz, cpx<T> N,
                                                const Rk4Ops<T>& o, bool store = true) {
@@ -577,24 +584,24 @@ __device__ __forceinline__ cpx<T> rk4_finish_L(const StageArgs<T>& a, long long 
   if (stage == 0) {
     const cpx<T> nx = {x.re + h2 * rhs.re, x.im + h2 * rhs.im};
     if (store) {
-      if (a.store_x) st_nt(a.stg + ubase + off, nx);
-      st_nt(a.acc + ubase + off, rhs);
+      if (a.store_x) st_pol<NTS>(a.stg + ubase + off, nx);
+      st_pol<NTS>(a.acc + ubase + off, rhs);
     }
     return nx;
   } else if (stage == 1) {
     const cpx<T> nx = {o.s.re + h2 * rhs.re, o.s.im + h2 * rhs.im};
     if (store) {
-      st_nt(a.stg + ubase + off, nx);
-      st_nt(a.acc + ubase + off, {o.ac.re / (T)6 + rhs.re / (T)3, o.ac.im / (T)6 + rhs.im / (T)3});
+      st_pol<NTS>(a.stg + ubase + off, nx);
+      st_pol<NTS>(a.acc + ubase + off, {o.ac.re / (T)6 + rhs.re / (T)3, o.ac.im / (T)6 + rhs.im / (T)3});
     }
     return nx;
   } else if (stage == 2) {
     const cpx<T> nx = {o.s.re + dt * rhs.re, o.s.im + dt * rhs.im};
     const cpx<T> tot = {o.ac.re + rhs.re / (T)3, o.ac.im + rhs.im / (T)3};
     if (store) {
-      st_nt(a.stg + ubase + off, nx);
-      if (sizeof(T) == 4 && a.sol_lo) st_nt(a.acc + ubase + off, tot);
-      else st_nt(a.sol + ubase + off, {o.s.re + dt * tot.re, o.s.im + dt * tot.im});
+      st_pol<NTS>(a.stg + ubase + off, nx);
+      if (sizeof(T) == 4 && a.sol_lo) st_pol<NTS>(a.acc + ubase + off, tot);
+      else st_pol<NTS>(a.sol + ubase + off, {o.s.re + dt * tot.re, o.s.im + dt * tot.im});
     }
     return nx;
   } else {
@@ -605,7 +612,7 @@ __device__ __forceinline__ cpx<T> rk4_finish_L(const StageArgs<T>& a, long long 
         const T f = (T)filter_value(a.g, a.f_order, a.f_innerK, a.f_decay, ikx, iky, ikz);
         ns = {f * ns.re, f * ns.im};
       }
-      if (store) st_nt(a.sol + ubase + off, ns);
+      if (store) st_pol<NTS>(a.sol + ubase + off, ns);
       return ns;
     }
     // compensated accumulation: (hi, lo) += dt tot with TwoSum + renormalisation
@@ -623,8 +630,8 @@ __device__ __forceinline__ cpx<T> rk4_finish_L(const StageArgs<T>& a, long long 
       li_ = (T)(xi - (double)hi_);
     }
     if (store) {
-      st_nt(a.sol + ubase + off, {hr, hi_});
-      st_nt(a.sol_lo + ubase + off, {lr, li_});
+      st_pol<NTS>(a.sol + ubase + off, {hr, hi_});
+      st_pol<NTS>(a.sol_lo + ubase + off, {lr, li_});
     }
     return {hr, hi_};
   }
@@ -632,11 +639,11 @@ __device__ __forceinline__ cpx<T> rk4_finish_L(const StageArgs<T>& a, long long 
 
 // op = the member's operator coefficients (a.op[member]), read once by the caller: a
 // read through a.op between the state stores would be repeated for every element.
-template <typename T, int ST = -1>
+template <typename T, int ST = -1, bool NTS = true>
 __device__ __forceinline__ cpx<T> rk4_finish(const StageArgs<T>& a, long long ubase, uint32_t off,
                                              const OpParams& op, int ikx, int iky, int ikz, cpx<T> N,
                                              const Rk4Ops<T>& o, bool store = true) {
-  return rk4_finish_L<T, ST>(a, ubase, off, (T)linear_op(a.g, op, a.nkappah, ikx, iky, ikz), ikx,
+  return rk4_finish_L<T, ST, NTS>(a, ubase, off, (T)linear_op(a.g, op, a.nkappah, ikx, iky, ikz), ikx,
                              iky, ikz, N, o, store);
 }
 
@@ -691,7 +698,7 @@ __device__ __forceinline__ EtdOps<T> etd_load(const StageArgs<T>& a, long long u
   return o;
 }
 
-template <typename T, int ST = -1>
+template <typename T, int ST = -1, bool NTS = true>
 __device__ __forceinline__ cpx<T> etd_finish(const StageArgs<T>& a, long long ubase, uint32_t off,
                                              int ikx, int iky, int ikz, cpx<T> N,
                                              const EtdOps<T>& o, bool store = true) {
@@ -702,11 +709,11 @@ __device__ __forceinline__ cpx<T> etd_finish(const StageArgs<T>& a, long long ub
     const cpx<T> v = {(T)(o.e2 * o.sr + o.z * nre), (T)(o.e2 * o.si + o.z * nim)};
     if (store) {
       if (stage == 0) {
-        st_nt(a.stg + ubase + off, v);
-        st_nt(a.n1 + ubase + off, N);
+        st_pol<NTS>(a.stg + ubase + off, v);
+        st_pol<NTS>(a.n1 + ubase + off, N);
       } else {
-        st_nt(a.stg2 + ubase + off, v);
-        st_nt(a.n2 + ubase + off, N);
+        st_pol<NTS>(a.stg2 + ubase + off, v);
+        st_pol<NTS>(a.n2 + ubase + off, N);
       }
     }
     return v;
@@ -718,10 +725,10 @@ __device__ __forceinline__ cpx<T> etd_finish(const StageArgs<T>& a, long long ub
     const double ai = o.e * o.si + o.al * (double)o.n1.im + be2 * ((double)o.n2.im + nim);
     const cpx<T> hi = {(T)ar, (T)ai};
     if (store) {
-      st_nt(a.stg2 + ubase + off, v);
-      st_nt(a.acc + ubase + off, hi);
+      st_pol<NTS>(a.stg2 + ubase + off, v);
+      st_pol<NTS>(a.acc + ubase + off, hi);
       if (kMixed && a.sol_lo)   // the low part of acc
-        st_nt(a.n2 + ubase + off, {(T)(ar - (double)hi.re), (T)(ai - (double)hi.im)});
+        st_pol<NTS>(a.n2 + ubase + off, {(T)(ar - (double)hi.re), (T)(ai - (double)hi.im)});
     }
     return v;
   } else {
@@ -733,9 +740,9 @@ __device__ __forceinline__ cpx<T> etd_finish(const StageArgs<T>& a, long long ub
     }
     const cpx<T> hi = {(T)sr, (T)si};
     if (store) {
-      st_nt(a.sol + ubase + off, hi);
+      st_pol<NTS>(a.sol + ubase + off, hi);
       if (kMixed && a.sol_lo)
-        st_nt(a.sol_lo + ubase + off, {(T)(sr - (double)hi.re), (T)(si - (double)hi.im)});
+        st_pol<NTS>(a.sol_lo + ubase + off, {(T)(sr - (double)hi.re), (T)(si - (double)hi.im)});
     }
     return hi;
   }

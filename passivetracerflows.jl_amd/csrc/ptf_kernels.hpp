@@ -162,9 +162,10 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, TX * (N / R)),
 void col_kernel(ColArgs<T> a) {
   static_assert(epf_ok<MODE, EPF>(), "stage kernels take an epilogue family, others none");
   constexpr int P = N / R;
-  // Non-temporal loads only where a block covers whole 128-byte lines of a field: along
-  // the bricked axis a column's run is the 8 planes of a brick, so a block reads TX * 8
-  // consecutive elements; the columns of the 3D y pass are contiguous (TX elements).
+  // Non-temporal loads (and state stores) only where a block covers whole 128-byte lines
+  // of a field: along the bricked axis a column's run is the 8 planes of a brick, so a
+  // block reads TX * 8 consecutive elements; the columns of the 3D y pass are contiguous
+  // (TX elements).
   constexpr int kRun = MODE == COL_D2 ? TX : TX * 8;
   constexpr bool kNtS = kRun * (int)sizeof(cpx<T>) >= 128;   // spectral fields, state
   constexpr bool kNtC = kRun * (int)sizeof(double) >= 128;   // FP64 real coefficients
@@ -367,7 +368,7 @@ void col_kernel(ColArgs<T> a) {
               Rk4Ops<T> oq;
               if constexpr (kPre) oq = ops[q];
               else oq = o[i];
-              v[q] = rk4_finish<T, ST>(a.st, ubase, off(q), opm, ikx_col, iky, ikz, v[q], oq, valid);
+              v[q] = rk4_finish<T, ST, kNtS>(a.st, ubase, off(q), opm, ikx_col, iky, ikz, v[q], oq, valid);
             }
           }
         });
@@ -387,7 +388,7 @@ void col_kernel(ColArgs<T> a) {
               const int ia = j + q * P;
               const int iky = a.axis == 2 ? by : ia;
               const int ikz = a.axis == 2 ? ia : iklast_col;
-              v[q] = etd_finish<T, ST>(a.st, ubase, off(q), ikx_col, iky, ikz, v[q], o[i], valid);
+              v[q] = etd_finish<T, ST, kNtS>(a.st, ubase, off(q), ikx_col, iky, ikz, v[q], o[i], valid);
             }
           }
         });
@@ -464,7 +465,8 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, TX * (N / R)),
 void col_pipe_kernel(ColArgs<T> a) {
   static_assert(EPF != EPF_NONE, "a stage kernel takes an epilogue family");
   constexpr int P = N / R;
-  // non-temporal operand loads where a tile covers whole 128-byte lines (as in col_kernel)
+  // non-temporal operand loads and state stores where a tile covers whole 128-byte lines
+  // (as in col_kernel)
   constexpr bool kNtS = TX * 8 * (int)sizeof(cpx<T>) >= 128;
   constexpr bool kNtC = TX * 8 * (int)sizeof(double) >= 128;
   const int tiles_x = (a.ncols + TX - 1) / TX;
@@ -595,7 +597,7 @@ void col_pipe_kernel(ColArgs<T> a) {
           const int ia = j + q * P;
           const int iky = a.axis == 2 ? cur.by : ia;
           const int ikz = a.axis == 2 ? ia : iklast;
-          v[q] = rk4_finish<T, ST>(a.st, cur.ubase, off(cur, q), opm, ikx, iky, ikz, v[q], ops[q],
+          v[q] = rk4_finish<T, ST, kNtS>(a.st, cur.ubase, off(cur, q), opm, ikx, iky, ikz, v[q], ops[q],
                                    cur.valid);
         }
         next_stage_inverse(cur, v);
@@ -641,7 +643,7 @@ void col_pipe_kernel(ColArgs<T> a) {
             const int ia = j + q * P;
             const int iky = a.axis == 2 ? cur.by : ia;
             const int ikz = a.axis == 2 ? ia : iklast;
-            v[q] = etd_finish<T, ST>(a.st, cur.ubase, off(cur, q), ikx, iky, ikz, v[q], o[i], cur.valid);
+            v[q] = etd_finish<T, ST, kNtS>(a.st, cur.ubase, off(cur, q), ikx, iky, ikz, v[q], o[i], cur.valid);
           }
         }
         next_stage_inverse(cur, v);

@@ -51,3 +51,31 @@ def test_kernel_template_names_every_form():
     assert kt("row_advect2 N=512 R=4 W=8 occ=3 f32") == "row_kernel<float, 512, 4, 8, 2, 2, 3>"
     assert kt("row_advect3 N=256 R=4 W=8 par occ=3 f32") == \
         "row_adv_par_kernel<float, 256, 4, 8, 3, 3>"
+
+
+def test_retune_tunes_each_key_once(tmp_path, monkeypatch):
+    """bench.py --retune: the first problem of a workload drops the stored choice and tunes;
+    later problems of the same workload reuse what it stored."""
+    from passivetracerflows import tuning
+    path = str(tmp_path / "tuning.json")
+    tuning.store(path, {"k": [["slot", 4, 4, False, False, False]], "other": []})
+    monkeypatch.setattr(bench, "TUNING", path)
+    monkeypatch.setattr(bench, "_RETUNED", set())
+    seen = []
+
+    class Obj:
+        def tuning_key(self):
+            return "k"
+
+        def autotune(self, cache):
+            seen.append("k" in tuning.load(cache))
+            data = tuning.load(cache)
+            data.setdefault("k", [["slot", 8, 8, True, False, True]])
+            tuning.store(cache, data)
+            return "tuned" if not seen[-1] else "cache"
+
+    assert bench._tune(Obj(), "retune") == "tuned"     # stored choice dropped, tuned
+    assert bench._tune(Obj(), "retune") == "cache"     # the fresh choice reused
+    assert seen == [False, True]
+    assert tuning.load(path)["k"] == [["slot", 8, 8, True, False, True]]
+    assert "other" in tuning.load(path)
