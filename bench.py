@@ -44,7 +44,8 @@ PASS_MODEL_B_PER_GPS = 716.8
 
 def kernel_bytes(name: str, nS: int, nR: int, cb: int, rb: int, stage: int, stepper: str = "RK4"):
     """Algorithmic HBM bytes of one launch (pass model): every field a kernel must
-    read or write once.  cb/rb = bytes per complex/real element."""
+    read or write once.  cb/rb = bytes per complex/real element (cb = 8: the mixed form,
+    whose compensated state adds low parts)."""
     S, R = nS * cb, nR * rb
     C = nS * 8       # an FP64 ETDRK4 coefficient array (S-shaped, real)
     fam = name.split(" [")[0].split("_", 1)[1] if name.startswith("stage") else name.split(" [")[0]
@@ -68,10 +69,18 @@ def kernel_bytes(name: str, nS: int, nR: int, cb: int, rb: int, stage: int, step
     # ETDRK4: 0 and 1 read sol, E2, zeta and write the next input and N; 2 reads sol, sol1,
     # N1, N2, E2, zeta, E, alpha, beta and writes sol2 and the partial update; 3 reads the
     # partial update and Gamma and writes sol.
+    # The mixed form adds the low parts of its compensated pairs: ETDRK4 reads sol_lo in
+    # stages 0-2, writes acc's low part in stage 2 and reads it in stage 3, writes sol_lo
+    # in stage 3; RK4 stage 3 reads acc and sol_lo and writes sol_lo.
+    mixed = cb == 8
     if stepper.upper().endswith("ETDRK4"):
         state = {0: 3 * S + 2 * C, 1: 3 * S + 2 * C, 2: 6 * S + 5 * C, 3: 2 * S + C}[stage]
+        if mixed:
+            state += {0: S, 1: S, 2: 2 * S, 3: 2 * S}[stage]
     else:
         state = {0: 2 * S, 1: 4 * S, 2: 5 * S, 3: 3 * S}[stage]
+        if mixed and stage == 3:
+            state += 3 * S
     if fam in ("z_forward_stage", "y_forward_stage"):
         return S + state
     if fam in ("z_forward_stage_inverse", "y_forward_stage_inverse"):

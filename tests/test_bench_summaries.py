@@ -79,3 +79,19 @@ def test_retune_tunes_each_key_once(tmp_path, monkeypatch):
     assert seen == [False, True]
     assert tuning.load(path)["k"] == [["slot", 8, 8, True, False, True]]
     assert "other" in tuning.load(path)
+
+
+def test_stage_bytes_count_the_mixed_low_parts():
+    """Stage-kernel algorithmic bytes: FP64 RK4 14 S-fields of state per step, the mixed
+    form 17 (stage 4 reads acc and sol_lo and writes sol_lo); ETDRK4 mixed adds 6 low-part
+    fields per step.  Each fused stage kernel also moves its input and two outputs."""
+    nS, nR = 1000, 2000
+    name = "stage{}_y_forward_stage_inverse [x]"
+    def total(cb, stepper):
+        return sum(bench.kernel_bytes(name.format(s + 1), nS, nR, cb, cb // 2, s, stepper)
+                   for s in range(4))
+    assert total(16, "RK4") == (14 + 4 * 3) * nS * 16
+    assert total(8, "RK4") == (17 + 4 * 3) * nS * 8
+    C = nS * 8
+    assert total(16, "ETDRK4") == (14 + 4 * 3) * nS * 16 + 10 * C
+    assert total(8, "ETDRK4") == (20 + 4 * 3) * nS * 8 + 10 * C
