@@ -145,6 +145,17 @@ struct LayCol {
 template <int MODE>
 constexpr bool col_skew() { return MODE == COL_STAGE_D1; }
 
+// Elements per operand chunk of the ETDRK4 epilogue (the operands of a chunk are loaded
+// together: one memory round trip per chunk).  The third stage reads up to 9 operands per
+// element (26 VGPRs in FP64): 4 elements at a time, 2 under the 3-waves/SIMD budget.  The
+// others read 2 or 3 fields (6-8 VGPRs per element, after the transform has released its
+// registers): 8 elements at a time, 4 under the 3-waves/SIMD budget.
+template <int R, int ST, int OCC>
+constexpr int etd_chunk() {
+  const int c = ST == 2 ? (OCC > 0 ? 2 : 4) : (OCC > 0 ? 4 : 8);
+  return R < c ? R : c;
+}
+
 template <typename T, int N, int R, int TX, int MODE>
 constexpr int col_lds_line_elems() {
   return (N > R) ? (col_skew<MODE>() ? TX * col_line_stride<N>() : N * TX) : 1;
@@ -375,8 +386,7 @@ void col_kernel(ColArgs<T> a) {
       } else if (EPF != EPF_RK4 && a.st.kind == EPI_ETDRK4) {
         stage_dispatch([&](auto stc) {
           constexpr int ST = decltype(stc)::value;
-          // up to 9 operands per element: 4 elements at a time, 2 under the 3-waves/SIMD budget
-          constexpr int kChunk = OCC > 0 ? 2 : (R < 4 ? R : 4);
+          constexpr int kChunk = etd_chunk<R, ST, OCC>();
 #pragma unroll
           for (int q0 = 0; q0 < R; q0 += kChunk) {
             EtdOps<T> o[kChunk];
@@ -631,7 +641,7 @@ void col_pipe_kernel(ColArgs<T> a) {
         fft(std::integral_constant<int, -1>{}, v);
         const int ikx = cur.kx >> a.lbc;
         const int iklast = (cur.by << a.lbc) + (cur.kx & ((1 << a.lbc) - 1));
-        constexpr int kChunk = R < 4 ? R : 4;
+        constexpr int kChunk = etd_chunk<R, ST, 0>();
 #pragma unroll
         for (int q0 = 0; q0 < R; q0 += kChunk) {
           EtdOps<T> o[kChunk];
