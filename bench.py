@@ -312,9 +312,9 @@ def sync():
     call("ptf_device_synchronize")
 
 
-# ensemble warm-up steps: one block of 8 steps and one single step, so that both captured
+# ensemble warm-up steps: one block of 16 steps and one single step, so that both captured
 # step graphs exist before a timed region
-ENS_WARM = 9
+ENS_WARM = 17
 
 
 def time_steps(stepfn, steps):
@@ -586,7 +586,7 @@ def extra_forms(dev, steps, ens_steps, autotune="cache"):
             e.set_c(i, m["c0"])
         if prec == "mixed":
             _tune(e, autotune)
-        # both step graphs (a block of 8 steps, a single step) captured and replayed once
+        # both step graphs (a block of 16 steps, a single step) captured and replayed once
         # outside the timed region
         e.step(ENS_WARM, graph=True)
         k = min(ens_steps, ens_cfg["nsteps"]) - ENS_WARM

@@ -247,7 +247,7 @@ int ptf_ensemble_set_c(ptf_ensemble* ens, int32_t member, const double* c);
 int ptf_ensemble_get_c(ptf_ensemble* ens, int32_t member, double* c);
 int ptf_ensemble_get_sol(ptf_ensemble* ens, int32_t member, double* sol);
 /* Advance every member nsteps steps; use_graph != 0 replays captured hipGraphs (blocks
- * of 8 steps in which each size group runs ahead on its own, then single steps). */
+ * of 16 steps in which each size group runs ahead on its own, then single steps). */
 int ptf_ensemble_step(ptf_ensemble* ens, int64_t nsteps, int32_t use_graph);
 /* A member's clock (every member steps with the ensemble; t advances by that member's dt
  * after each step, as [FF] stepforward! advances prob.clock), and a shared reset. */

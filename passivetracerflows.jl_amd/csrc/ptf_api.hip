@@ -2486,7 +2486,7 @@ static int ensemble_record_steps(ptf_ensemble* e, int nsteps) {
 }
 
 // Steps per replay of the multi-step ensemble graph (the groups join once per block).
-static constexpr int kEnsGraphSteps = 8;
+static constexpr int kEnsGraphSteps = 16;
 
 // capture graph `which` (0: one step, 1: kEnsGraphSteps steps) if it is not there yet
 static int ensemble_graph(ptf_ensemble* ens, int which) {

@@ -60,13 +60,13 @@ def test_ensemble_fp64_and_mixed_vs_oracle(dev):
 
 
 def test_ensemble_graph_equals_direct(dev):
-    """Graph replay (blocks of 8 steps with each size group in its own chain, then single
+    """Graph replay (blocks of 16 steps with each size group in its own chain, then single
     steps) is bitwise the direct launches, across several calls with host work between."""
     dt = 5e-3
     members = _members()[:5]       # groups of 64^2, 128^2 and 256^2 members
     a = _ens(members, "fp64", dt)
     b = _ens(members, "fp64", dt)
-    for n in (11, 9, 3, 16):       # 1 block + 3, 1 block + 1, 3 single steps, 2 blocks
+    for n in (19, 17, 3, 32):      # 1 block + 3, 1 block + 1, 3 single steps, 2 blocks
         a.step(n, graph=True)
         b.step(n, graph=False)
         for i in range(len(members)):
