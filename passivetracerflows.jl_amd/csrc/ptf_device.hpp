@@ -292,7 +292,7 @@ __device__ __forceinline__ void lds_sync(bool wave) {
 // One pass of a line FFT (see the header): the E/RP butterflies of this thread on its
 // natural-layout registers a (already read from LDS unless this is the first pass),
 // outputs stored to LDS at their Stockham positions unless this is the last pass.
-template <typename T, int N, int E, int DIR, int PASS, class Lay>
+template <typename T, int N, int E, int DIR, int PASS, class Lay, bool TWH = false>
 __device__ __forceinline__ void fft_pass(cpx<T> (&a)[E], cpx<T>* lds, const Lay& lay, int c, int j,
                                          const cpx<T>* __restrict__ tw, int twstride, bool wave) {
   constexpr RadixPlan pl = radix_plan(N, E);
@@ -320,7 +320,16 @@ __device__ __forceinline__ void fft_pass(cpx<T> (&a)[E], cpx<T>* lds, const Lay&
       const int step = (N / (Ns * RP)) * twstride;
 #pragma unroll
       for (int r = 1; r < RP; ++r) {
-        cpx<T> w = tw[r * m * step];
+        cpx<T> w;
+        if constexpr (TWH) {
+          // half table (twstride 1): tw[k] for k < N/2, the rest is -tw[k - N/2]
+          const int k = r * m * step;
+          const bool hi = k >= N / 2;
+          w = tw[hi ? k - N / 2 : k];
+          if (hi) w = {-w.re, -w.im};
+        } else {
+          w = tw[r * m * step];
+        }
         if constexpr (DIR > 0) w.im = -w.im;
         t[r] = cmul(t[r], w);
       }
@@ -344,13 +353,13 @@ __device__ __forceinline__ void fft_pass(cpx<T> (&a)[E], cpx<T>* lds, const Lay&
   }
 }
 
-template <typename T, int N, int E, int DIR, int PASS, class Lay>
+template <typename T, int N, int E, int DIR, int PASS, class Lay, bool TWH = false>
 __device__ __forceinline__ void fft_passes(cpx<T> (&a)[E], cpx<T>* lds, const Lay& lay, int c, int j,
                                            const cpx<T>* __restrict__ tw, int twstride, bool wave) {
   constexpr RadixPlan pl = radix_plan(N, E);
   if constexpr (PASS < pl.n) {
-    fft_pass<T, N, E, DIR, PASS, Lay>(a, lds, lay, c, j, tw, twstride, wave);
-    fft_passes<T, N, E, DIR, PASS + 1, Lay>(a, lds, lay, c, j, tw, twstride, wave);
+    fft_pass<T, N, E, DIR, PASS, Lay, TWH>(a, lds, lay, c, j, tw, twstride, wave);
+    fft_passes<T, N, E, DIR, PASS + 1, Lay, TWH>(a, lds, lay, c, j, tw, twstride, wave);
   }
 }
 
@@ -360,17 +369,19 @@ __device__ __forceinline__ void fft_passes(cpx<T> (&a)[E], cpx<T>* lds, const La
 // read (callers that write LDS afterwards must barrier first); every thread of the block
 // must call it.  wave = true (every line of the block within one wavefront, lines never
 // sharing LDS entries) orders the exchanges per wavefront instead (lds_sync); then a
-// twiddle table in LDS must be complete before the call.
-template <typename T, int N, int R, int DIR, class Lay>
+// twiddle table in LDS must be complete before the call.  TWH: tw holds only the first
+// N/2 entries (twstride 1), the others follow from tw[k + N/2] = -tw[k].
+template <typename T, int N, int R, int DIR, class Lay, bool TWH = false>
 __device__ __forceinline__ void line_fft(cpx<T> (&a)[R], cpx<T>* lds, const Lay& lay, int c, int j,
                                          const cpx<T>* __restrict__ tw, int twstride,
                                          bool wave = false) {
   constexpr RadixPlan pl = radix_plan(N, R);
   static_assert(pl.n > 0, "N must split into radices that divide R");
+  static_assert(!TWH || N % 2 == 0, "a half twiddle table needs an even length");
   if constexpr (pl.n == 1) {
     dft<R, DIR>(a);
   } else {
-    fft_passes<T, N, R, DIR, 0, Lay>(a, lds, lay, c, j, tw, twstride, wave);
+    fft_passes<T, N, R, DIR, 0, Lay, TWH>(a, lds, lay, c, j, tw, twstride, wave);
   }
 }
 

@@ -160,9 +160,15 @@ template <typename T, int N, int R, int TX, int MODE>
 constexpr int col_lds_line_elems() {
   return (N > R) ? (col_skew<MODE>() ? TX * col_line_stride<N>() : N * TX) : 1;
 }
+// FP64 lines of 2048 points keep half of the table in LDS (w[k + N/2] = -w[k]): 16 instead
+// of 32 KB, so that three blocks of a fused stage kernel fit a CU instead of two
+template <typename T, int N, int R>
+constexpr bool col_tw_half() { return col_tw_lds<N, R>() && N * (int)sizeof(cpx<T>) > 16384; }
+template <typename T, int N, int R>
+constexpr int col_tw_elems() { return col_tw_lds<N, R>() ? (col_tw_half<T, N, R>() ? N / 2 : N) : 0; }
 template <typename T, int N, int R, int TX, int MODE>
 constexpr int col_lds_elems() {
-  return col_lds_line_elems<T, N, R, TX, MODE>() + (col_tw_lds<N, R>() ? N : 0);
+  return col_lds_line_elems<T, N, R, TX, MODE>() + col_tw_elems<T, N, R>();
 }
 
 template <typename T, int N, int R, int TX, int MODE, int DIR, int OCC = 0, int EPF = EPF_NONE>
@@ -183,7 +189,8 @@ void col_kernel(ColArgs<T> a) {
   using Lay = LayCol<N, TX, col_skew<MODE>()>;
   const Lay lay{};
   __shared__ cpx<T> lds[col_lds_line_elems<T, N, R, TX, MODE>()];
-  __shared__ cpx<T> twl[col_tw_lds<N, R>() ? N : 1];
+  constexpr bool kTwh = col_tw_half<T, N, R>();
+  __shared__ cpx<T> twl[col_tw_lds<N, R>() ? col_tw_elems<T, N, R>() : 1];
   const int t = threadIdx.x;
   const int lb = a.lb;
   // Lines within one wavefront: along a bricked axis, when the P threads of a line divide
@@ -193,7 +200,7 @@ void col_kernel(ColArgs<T> a) {
   if constexpr (col_tw_lds<N, R>()) {
     // twl[m] = exp(-2 pi i m / N); the first barrier of every transform orders these
     // writes before its first twiddle read (with per-wavefront exchanges, this barrier)
-    for (int i = t; i < N; i += TX * P) twl[i] = a.tw[i * a.twstride];
+    for (int i = t; i < col_tw_elems<T, N, R>(); i += TX * P) twl[i] = a.tw[i * a.twstride];
     if (wave) __syncthreads();
   }
   // lane t = rb + 2^lb (c + TX jh): the brick's planes vary fastest, then the
@@ -209,7 +216,7 @@ void col_kernel(ColArgs<T> a) {
   // the line transform in direction D with this block's twiddle source
   auto fft = [&](auto dir_c, cpx<T>(&x)[R]) {
     constexpr int D = decltype(dir_c)::value;
-    if constexpr (col_tw_lds<N, R>()) line_fft<T, N, R, D, Lay>(x, lds, lay, c, j, twl, 1, wave);
+    if constexpr (col_tw_lds<N, R>()) line_fft<T, N, R, D, Lay, kTwh>(x, lds, lay, c, j, twl, 1, wave);
     else line_fft<T, N, R, D, Lay>(x, lds, lay, c, j, a.tw, a.twstride, wave);
   };
   int bx, by, member;

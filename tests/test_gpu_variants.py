@@ -60,6 +60,8 @@ CASES = [
     # (width, radix) variants that still fit the 160 KiB LDS budget
     (2, (2048, 64), "RK4", np.float64), (2, (64, 2048), "ETDRK4", np.float32),
     (3, (16, 16, 1024), "RK4", np.float64), (2, (1024, 64), "ETDRK4", np.float32),
+    # FP64 2048-point columns keep half of their twiddle table in LDS (w[k + N/2] = -w[k])
+    (2, (64, 2048), "ETDRK4", np.float64), (2, (64, 2048), "RK4", np.float64),
 ]
 
 
