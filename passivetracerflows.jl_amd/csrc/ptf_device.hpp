@@ -322,10 +322,11 @@ __device__ __forceinline__ void fft_pass(cpx<T> (&a)[E], cpx<T>* lds, const Lay&
       for (int r = 1; r < RP; ++r) {
         cpx<T> w;
         if constexpr (TWH) {
-          // half table (twstride 1): tw[k] for k < N/2, the rest is -tw[k - N/2]
+          // half table: tw[k] for k < LT/2 (LT = N twstride), the rest is -tw[k - LT/2]
           const int k = r * m * step;
-          const bool hi = k >= N / 2;
-          w = tw[hi ? k - N / 2 : k];
+          const int half = (N / 2) * twstride;
+          const bool hi = k >= half;
+          w = tw[hi ? k - half : k];
           if (hi) w = {-w.re, -w.im};
         } else {
           w = tw[r * m * step];
@@ -370,7 +371,7 @@ __device__ __forceinline__ void fft_passes(cpx<T> (&a)[E], cpx<T>* lds, const La
 // must call it.  wave = true (every line of the block within one wavefront, lines never
 // sharing LDS entries) orders the exchanges per wavefront instead (lds_sync); then a
 // twiddle table in LDS must be complete before the call.  TWH: tw holds only the first
-// N/2 entries (twstride 1), the others follow from tw[k + N/2] = -tw[k].
+// LT/2 entries, the others follow from tw[k + LT/2] = -tw[k].
 template <typename T, int N, int R, int DIR, class Lay, bool TWH = false>
 __device__ __forceinline__ void line_fft(cpx<T> (&a)[R], cpx<T>* lds, const Lay& lay, int c, int j,
                                          const cpx<T>* __restrict__ tw, int twstride,

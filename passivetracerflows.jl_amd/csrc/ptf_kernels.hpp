@@ -795,8 +795,14 @@ __device__ __forceinline__ long long brick_row_base(long long row0, int lb, int 
 // the first nx/2).
 template <int NX>
 constexpr bool row_tw_lds() { return NX <= 2048; }
+// FP64 rows of 2048 points keep half of the table (w[k + nx/2] = -w[k]; the pre- and
+// post-processing read only k < nx/2): 16 instead of 32 KB of LDS per block
 template <typename T, int NX>
-constexpr int row_tw_bytes() { return row_tw_lds<NX>() ? NX * (int)sizeof(cpx<T>) : 0; }
+constexpr bool row_tw_half() { return row_tw_lds<NX>() && NX * (int)sizeof(cpx<T>) > 16384; }
+template <typename T, int NX>
+constexpr int row_tw_elems() { return row_tw_lds<NX>() ? (row_tw_half<T, NX>() ? NX / 2 : NX) : 0; }
+template <typename T, int NX>
+constexpr int row_tw_bytes() { return row_tw_elems<T, NX>() * (int)sizeof(cpx<T>); }
 
 template <typename T, int NX, int R, int NL, int MODE, int NIN, int OCC = 0>
 __global__ __attribute__((amdgpu_flat_work_group_size(1, NL * ((NX / 2) / R)),
@@ -814,11 +820,12 @@ void row_kernel(RowArgs<T> a) {
   using Lay = LayStrided;
   const Lay lay = a.lb ? Lay{1, NL} : Lay{M + PAD, 1};
   __shared__ cpx<T> lds[NL * (M + PAD)];
-  __shared__ cpx<T> twl[row_tw_lds<NX>() ? NX : 1];
+  constexpr bool kTwh = row_tw_half<T, NX>();
+  __shared__ cpx<T> twl[row_tw_lds<NX>() ? row_tw_elems<T, NX>() : 1];
   const int t = threadIdx.x;
   if constexpr (row_tw_lds<NX>()) {
     // read only after the barriers of the first row exchange or transform
-    for (int i = t; i < NX; i += (int)blockDim.x) twl[i] = a.tw[i * a.twstride];
+    for (int i = t; i < row_tw_elems<T, NX>(); i += (int)blockDim.x) twl[i] = a.tw[i * a.twstride];
   }
   // lane t = rb + 2^lt (j + P g), 2^lt = min(2^lb, NL): the block's rows of a brick vary
   // fastest.  With part of a brick per block (NL < 2^lb) the sibling blocks of a brick,
@@ -858,7 +865,7 @@ void row_kernel(RowArgs<T> a) {
       const int k = j + q * P;
       z[q] = c2r_pre(z[q], zm[q], (row_tw_lds<NX>() ? twl[k] : a.tw[k * a.twstride]), s, k == 0);
     }
-    if constexpr (row_tw_lds<NX>()) line_fft<T, M, R, +1, Lay>(z, lds, lay, c, j, twl, 2);
+    if constexpr (row_tw_lds<NX>()) line_fft<T, M, R, +1, Lay, kTwh>(z, lds, lay, c, j, twl, 2);
     else line_fft<T, M, R, +1, Lay>(z, lds, lay, c, j, a.tw, 2 * a.twstride);
     if (valid) {
       cpx<T>* out = reinterpret_cast<cpx<T>*>(a.rout + pub);
@@ -913,7 +920,7 @@ void row_kernel(RowArgs<T> a) {
           }
           z[q] = c2r_pre(Xk, Xm, (row_tw_lds<NX>() ? twl[k] : a.tw[k * a.twstride]), s, k == 0);
         }
-        if constexpr (row_tw_lds<NX>()) line_fft<T, M, R, +1, Lay>(z, lds, lay, c, j, twl, 2);
+        if constexpr (row_tw_lds<NX>()) line_fft<T, M, R, +1, Lay, kTwh>(z, lds, lay, c, j, twl, 2);
         else line_fft<T, M, R, +1, Lay>(z, lds, lay, c, j, a.tw, 2 * a.twstride);
 #pragma unroll
         for (int q = 0; q < R; ++q) {
@@ -929,7 +936,7 @@ void row_kernel(RowArgs<T> a) {
       }
     }
     // forward half-length FFT of f[2m] + i f[2m+1]
-    if constexpr (row_tw_lds<NX>()) line_fft<T, M, R, -1, Lay>(f, lds, lay, c, j, twl, 2);
+    if constexpr (row_tw_lds<NX>()) line_fft<T, M, R, -1, Lay, kTwh>(f, lds, lay, c, j, twl, 2);
     else line_fft<T, M, R, -1, Lay>(f, lds, lay, c, j, a.tw, 2 * a.twstride);
     // exchange Z[M-k] through LDS
     __syncthreads();
@@ -1004,10 +1011,11 @@ void row_adv_par_kernel(RowArgs<T> a) {
   using Lay = LayStrided;
   const Lay lay = a.lb ? Lay{1, NL} : Lay{M + 1, 1};
   __shared__ cpx<T> lds_all[NIN * NL * (M + 1)];
-  __shared__ cpx<T> twl[row_tw_lds<NX>() ? NX : 1];
+  constexpr bool kTwh = row_tw_half<T, NX>();
+  __shared__ cpx<T> twl[row_tw_lds<NX>() ? row_tw_elems<T, NX>() : 1];
   if constexpr (row_tw_lds<NX>()) {
     // every group fills (before any group ends); read after the first row exchange
-    for (int i = (int)threadIdx.x; i < NX; i += (int)blockDim.x) twl[i] = a.tw[i * a.twstride];
+    for (int i = (int)threadIdx.x; i < row_tw_elems<T, NX>(); i += (int)blockDim.x) twl[i] = a.tw[i * a.twstride];
   }
   const int grp = (int)threadIdx.x / G;   // uniform over a wavefront
   const int t = (int)threadIdx.x - grp * G;
@@ -1058,7 +1066,7 @@ void row_adv_par_kernel(RowArgs<T> a) {
     }
     z[q] = c2r_pre(Xk, Xm, (row_tw_lds<NX>() ? twl[k] : a.tw[k * a.twstride]), s, k == 0);
   }
-  if constexpr (row_tw_lds<NX>()) line_fft<T, M, R, +1, Lay>(z, lds, lay, c, j, twl, 2);
+  if constexpr (row_tw_lds<NX>()) line_fft<T, M, R, +1, Lay, kTwh>(z, lds, lay, c, j, twl, 2);
   else line_fft<T, M, R, +1, Lay>(z, lds, lay, c, j, a.tw, 2 * a.twstride);
   cpx<T> f[R];
 #pragma unroll
@@ -1088,7 +1096,7 @@ void row_adv_par_kernel(RowArgs<T> a) {
     f[q] = v;
   }
   // forward half-length FFT of f[2m] + i f[2m+1] (the other groups match its barriers)
-  if constexpr (row_tw_lds<NX>()) line_fft<T, M, R, -1, Lay>(f, lds, lay, c, j, twl, 2);
+  if constexpr (row_tw_lds<NX>()) line_fft<T, M, R, -1, Lay, kTwh>(f, lds, lay, c, j, twl, 2);
   else line_fft<T, M, R, -1, Lay>(f, lds, lay, c, j, a.tw, 2 * a.twstride);
   __syncthreads();
 #pragma unroll
