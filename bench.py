@@ -584,8 +584,7 @@ def extra_forms(dev, steps, ens_steps, autotune="cache"):
         for i, m in enumerate(members):
             e.set_flow(i, *m["flow"])
             e.set_c(i, m["c0"])
-        if prec == "mixed":
-            _tune(e, autotune)
+        _tune(e, autotune)
         # both step graphs (a block of 16 steps, a single step) captured and replayed once
         # outside the timed region
         e.step(ENS_WARM, graph=True)
@@ -601,6 +600,7 @@ def extra_forms(dev, steps, ens_steps, autotune="cache"):
     for i, m in enumerate(members):
         e64r.set_flow(i, *[r32(f) for f in m["flow"]])
         e64r.set_c(i, r32(m["c0"]))
+    _tune(e64r, autotune)   # the same stored variants as the FP64 run it is compared with
     e64r.step(k + ENS_WARM, graph=True)
     out["ensemble512_fp32_graph"] = dict(value=gps(npts, k, s32), ms_per_step=1e3 * s32 / k, steps=k,
                                          points_per_step=npts)
